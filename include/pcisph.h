@@ -1,0 +1,214 @@
+/*
+ * pcisph.h - C ABI of the MI355X (gfx950) PCISPH simulation step.
+ *
+ * Drop-in for the reference's GPU step `sim::Fluid` (PCISPH/src/sim/Fluid.h:13-102,
+ * PCISPH/src/sim/Fluid.cpp:32-466).  One opaque handle replaces the class; one
+ * `pcisph_set_*` per reference setter with the same semantics; `pcisph_step`
+ * replaces `Fluid::update()`.  Plain pointers and sizes only: no HIP or torch
+ * types cross this boundary, no C++ exception escapes it (every entry point
+ * returns a pcisph_status and `pcisph_last_error` gives the message).
+ *
+ * Particle buffers use the reference layout (PCISPH/src/scenes.cpp:192-208,
+ * PCISPH/src/sim/Fluid.h:40-50): vectors are packed float[3N] (x0 y0 z0 x1 ...),
+ * scalars float[N].  After every step the fluid arrays are in the step's sorted
+ * (cell-hash) order, exactly as the reference permutes them in place each step
+ * (PCISPH/data/kernels/sort_utils.cl:51-78).
+ *
+ * Threading: a handle is used by one host thread at a time; its work is ordered
+ * on its own HIP stream.  Not re-entrant (neither is the reference,
+ * PCISPH/src/sim/Fluid.cpp:108-327).
+ */
+#ifndef PCISPH_H
+#define PCISPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pcisph_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pcisph_status {
+    PCISPH_OK = 0,
+    PCISPH_ERR_INVALID_ARGUMENT = 1, /* null handle/pointer, unknown field, bad option */
+    PCISPH_ERR_SIZE = 2,             /* buffer size inconsistent with the particle counts
+                                        (reference: std::runtime_error from
+                                        checkBuffersConsistent, Fluid.cpp:90-106) */
+    PCISPH_ERR_DEVICE = 3,           /* HIP runtime error (reference: cl::Error) */
+    PCISPH_ERR_NO_DEVICE = 4,        /* no usable gfx950 device / kernels missing */
+    PCISPH_ERR_OUT_OF_MEMORY = 5,
+    PCISPH_ERR_NUMERIC = 6,          /* a non-finite value appeared in the step */
+    PCISPH_ERR_INTERNAL = 7          /* a device-side bound was hit (e.g. sort spin limit) */
+} pcisph_status;
+
+/* Every particle array the step reads or writes.  Counts are in elements of the
+ * element type (float or uint32), i.e. 3*N for vector fields. */
+typedef enum pcisph_field {
+    PCISPH_FLUID_POSITIONS = 0,          /* float[3N]  Fluid.h:43 */
+    PCISPH_FLUID_VELOCITIES = 1,         /* float[3N]  Fluid.h:48 */
+    PCISPH_FLUID_NORMALS = 2,            /* float[3N]  Fluid.h:44 */
+    PCISPH_FLUID_PREDICTED_POSITIONS = 3,/* float[3N]  Fluid.h:45 */
+    PCISPH_FLUID_DENSITIES = 4,          /* float[N]   Fluid.h:46 */
+    PCISPH_FLUID_OTHER_FORCES = 5,       /* float[3N]  Fluid.h:47 */
+    PCISPH_FLUID_PRESSURES = 6,          /* float[N]   Fluid.h:49 */
+    PCISPH_FLUID_PRESSURE_FORCES = 7,    /* float[3N]  Fluid.h:50 */
+    PCISPH_FLUID_DENSITY_VARIATIONS = 8, /* float[N]   Fluid.h:99 (internal there) */
+    PCISPH_BOUNDARY_POSITIONS = 9,       /* float[3Nb] Fluid.h:40 */
+    PCISPH_BOUNDARY_PRESSURES = 10,      /* float[Nb]  Fluid.h:41 */
+    /* sort products, exported for parity checks (internal buffers in the reference,
+       Fluid.h:88-98) */
+    PCISPH_FLUID_KEYS = 11,              /* uint32[N]  sorted cell-hash keys */
+    PCISPH_FLUID_SRC_LOCATIONS = 12,     /* uint32[N]  permutation: sorted slot -> previous slot */
+    PCISPH_FLUID_CELL_OFFSETS = 13,      /* uint32[2B] (start, end) per bucket, (0,0) if empty */
+    PCISPH_BOUNDARY_KEYS = 14,           /* uint32[Nb] */
+    PCISPH_BOUNDARY_SRC_LOCATIONS = 15,  /* uint32[Nb] */
+    PCISPH_BOUNDARY_CELL_OFFSETS = 16,   /* uint32[2B] */
+    PCISPH_BOUNDARY_INIT_PRED_DENSITIES = 17, /* float[Nb]  (PCISPH/data/kernels/pcisph.cl:195-210) */
+    /* per-particle neighbour statistics of the last step's density pass:
+       (fluid candidates, boundary candidates, fluid neighbours with r^2 <= h^2,
+        boundary neighbours with r^2 <= h^2).  Candidates = particles in the 27
+       looked-up buckets (PCISPH/data/kernels/grid_utils.cl:65-79). */
+    PCISPH_FLUID_NEIGHBOR_COUNTS = 18,   /* uint32[4N] */
+    PCISPH_FIELD_COUNT = 19
+} pcisph_field;
+
+/* Options the reference hard-codes (PCISPH/src/sim/Fluid.cpp:250-251,118). */
+typedef struct pcisph_solver_options {
+    uint32_t min_iterations;        /* reference: 3 (termination tested from i >= 2) */
+    uint32_t max_iterations;        /* reference: 7 */
+    uint32_t launch_boundary_init;  /* 1: run the boundary-boundary predicted density pass
+                                       (pcisph.cl:195-210) after each boundary sort; the
+                                       reference sets its arguments but never launches it
+                                       (Fluid.cpp:166-171), leaving the buffer as allocated:
+                                       0 (default) keeps it zero. */
+    uint32_t resort_boundary;       /* 1: re-sort boundary particles every step (moving walls).
+                                       The reference sorts them only after a parameter change
+                                       (Fluid.cpp:135,173). */
+    uint32_t block_size;            /* threads per block of the per-particle kernels;
+                                       0 = autotuned/default */
+    uint32_t sort_digit_bits;       /* radix digit width, 4..8; 0 = autotuned/default */
+    uint32_t reserved[2];
+} pcisph_solver_options;
+
+/* Per-step report (the reference only has a compiled-out timer, Fluid.cpp:319-326). */
+typedef struct pcisph_step_stats {
+    uint32_t iterations;            /* correction iterations run in this step */
+    uint32_t converged;             /* 1: stopped on max dv / rho0 < threshold */
+    float max_density_variation;    /* max over fluid of max(0, rho* - rho0), last iteration */
+    float max_density_error;        /* max_density_variation / rest_density (FP32, as tested) */
+    uint32_t nonfinite;             /* 1 if a non-finite position/velocity was produced */
+    uint32_t reserved;
+    uint64_t fluid_candidates;      /* sum over fluid particles of fluid candidates */
+    uint64_t boundary_candidates;   /* sum over fluid particles of boundary candidates */
+} pcisph_step_stats;
+
+typedef struct pcisph_solver pcisph_solver;
+
+/* Fluid::Fluid(ctx, device, queue) (Fluid.cpp:32-88).  `hip_stream` is a hipStream_t
+ * or NULL (the solver then creates its own stream on `device_ordinal`). */
+int pcisph_create(int device_ordinal, void* hip_stream, pcisph_solver** out);
+int pcisph_destroy(pcisph_solver* s);
+/* Message of the last failing call on this handle ("" if none).  With a NULL
+ * handle: the last pcisph_create failure of this thread. */
+const char* pcisph_last_error(const pcisph_solver* s);
+
+/* Setters (Fluid.h:21-29, Fluid.cpp:333-375).  All but the threshold mark the
+ * parameters dirty: the next step recomputes the derived parameters
+ * (Fluid.cpp:377-443), reallocates internal buffers, re-sorts the boundary and
+ * ZEROES the fluid velocities (Fluid.cpp:446-466), as the reference does. */
+int pcisph_set_boundary_count(pcisph_solver* s, uint32_t boundary_count);
+int pcisph_set_fluid_count(pcisph_solver* s, uint32_t fluid_count);
+int pcisph_set_delta_t(pcisph_solver* s, float delta_t);
+int pcisph_set_rest_density(pcisph_solver* s, float rest_density);
+int pcisph_set_particle_radius(pcisph_solver* s, float particle_radius);
+int pcisph_set_gravity(pcisph_solver* s, float gravity);
+int pcisph_set_viscosity(pcisph_solver* s, float viscosity);
+int pcisph_set_surface_tension(pcisph_solver* s, float surface_tension_coefficient);
+int pcisph_set_density_variation_threshold(pcisph_solver* s, float threshold);
+
+/* Fluid::get_params (Fluid.cpp:329-331): the byte-identical 80-byte struct.
+ * Derived fields are those of the last step (as in the reference). */
+int pcisph_get_params(const pcisph_solver* s, Simulation_Params* out);
+int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt);
+int pcisph_get_solver_options(const pcisph_solver* s, pcisph_solver_options* out);
+
+/* Fluid::checkBuffersConsistent (Fluid.cpp:90-106): PCISPH_ERR_SIZE if a buffer
+ * does not match the counts. */
+int pcisph_check_buffers_consistent(const pcisph_solver* s);
+
+/* Arithmetic of the step (no counterpart in the reference, whose precision is the
+ * OpenCL device's float):
+ *   0 PCISPH_PRECISION_FAST  FP32 with fused multiply-adds and hardware reciprocal /
+ *                            square root (default; the accuracy class OpenCL allows
+ *                            for `/`, distance() and fast_length());
+ *   1 PCISPH_PRECISION_IEEE  FP32, one rounding per operation, the reference's order
+ *                            of operations (matches the CPU restatement bit for bit);
+ *   2 PCISPH_PRECISION_FP64  the same operations in double precision.
+ * Switching to or from FP64 reallocates the particle arrays: set it before uploading. */
+enum { PCISPH_PRECISION_FAST = 0, PCISPH_PRECISION_IEEE = 1, PCISPH_PRECISION_FP64 = 2 };
+int pcisph_set_precision(pcisph_solver* s, int precision);
+
+/* Element count the solver expects for `field` under the current counts. */
+int pcisph_field_size(const pcisph_solver* s, int field, size_t* out_count);
+
+/* Copy a field in or out in the reference layout.  `count` must equal
+ * pcisph_field_size (PCISPH_ERR_SIZE otherwise).  Host pointers; downloads wait
+ * for the step to finish.  Uploading PCISPH_BOUNDARY_POSITIONS schedules a
+ * boundary re-sort at the next step.  Sort products and neighbour counts are
+ * read-only. */
+int pcisph_upload(pcisph_solver* s, int field, const void* host, size_t count);
+int pcisph_download(pcisph_solver* s, int field, void* host, size_t count);
+/* Same, with a device pointer on the solver's device (ordered on its stream). */
+int pcisph_upload_device(pcisph_solver* s, int field, const void* device_ptr, size_t count);
+int pcisph_download_device(pcisph_solver* s, int field, void* device_ptr, size_t count);
+
+/* Float fields widened to double (exact for FP32 solvers; the FP64 solver's own
+ * values otherwise), for error reports against an FP64 run. */
+int pcisph_download_f64(pcisph_solver* s, int field, double* host, size_t count);
+
+/* Fluid::update() (Fluid.cpp:108-327): one simulation step.  `stats` may be NULL.
+ * Returns when the correction loop's iteration count is known; the final
+ * integration may still be running (pcisph_sync waits). */
+int pcisph_step(pcisph_solver* s, pcisph_step_stats* stats);
+/* n_steps consecutive steps; per-step stats into stats[0..n_steps) if non-NULL. */
+int pcisph_run(pcisph_solver* s, uint32_t n_steps, pcisph_step_stats* stats);
+int pcisph_sync(pcisph_solver* s);
+
+/* Per-stage timing with HIP events on the solver's stream (the reference's
+ * compiled-out step timer, Fluid.cpp:123,182,316,319-326, per stage).  Enabling
+ * resets the totals; iterations enqueued ahead of the loop's end and then skipped
+ * are not counted. */
+typedef enum pcisph_stage {
+    PCISPH_STAGE_BOUNDARY_SORT = 0, /* hash + sort + reorder of the boundary */
+    PCISPH_STAGE_HASH = 1,          /* cell-table reset + hash keys (sort_utils.cl:7-23) */
+    PCISPH_STAGE_SORT = 2,          /* radix sort, all passes */
+    PCISPH_STAGE_REORDER = 3,       /* sort_utils.cl:51-78 */
+    PCISPH_STAGE_DENSITY = 4,       /* pcisph.cl:65-92 (+ boundary pressure reset) */
+    PCISPH_STAGE_NORMAL = 5,        /* pcisph.cl:94-112 */
+    PCISPH_STAGE_FORCE_INIT = 6,    /* pcisph.cl:121-173 + first prediction */
+    PCISPH_STAGE_PRESSURE = 7,      /* pcisph.cl:212-263, boundary and fluid */
+    PCISPH_STAGE_PRESSURE_FORCE = 8,/* pcisph.cl:265-300 + prediction / final integration */
+    PCISPH_STAGE_COUNT = 9
+} pcisph_stage;
+
+typedef struct pcisph_stage_times {
+    double ms[16];        /* summed device time per stage */
+    uint64_t launches[16];/* timed launches per stage */
+} pcisph_stage_times;
+
+int pcisph_profile_enable(pcisph_solver* s, int enable);
+/* Waits for the stream, then returns the totals since profiling was enabled. */
+int pcisph_profile_read(pcisph_solver* s, pcisph_stage_times* out);
+
+/* Derived parameters for a parameter set without a device (host only). */
+int pcisph_compute_params(float particle_radius, float rest_density, uint32_t boundary_count,
+                          uint32_t fluid_count, float viscosity, float surface_tension,
+                          float gravity, float delta_t, Simulation_Params* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCISPH_H */

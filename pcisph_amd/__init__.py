@@ -1,0 +1,152 @@
+"""pcisph_amd - the PCISPH simulation step on AMD Instinct MI355X (gfx950).
+
+`Fluid` mirrors the reference's `sim::Fluid` (PCISPH/src/sim/Fluid.h:13-102) over the C
+ABI of include/pcisph.h, implemented by libpcisph_amd.so (hand-written HIP kernels).
+There is no fallback: if the library is missing or no gfx950 device is present,
+constructing a Fluid raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (Field, PcisphError, SimulationParams, SolverBase, SolverOptions, StepStats,
+                  field_dtype)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpcisph_amd.so")
+
+PRECISION_FAST = 0
+PRECISION_IEEE = 1
+PRECISION_FP64 = 2
+
+EXPORTS = (
+    "pcisph_create", "pcisph_destroy", "pcisph_last_error",
+    "pcisph_set_boundary_count", "pcisph_set_fluid_count", "pcisph_set_delta_t",
+    "pcisph_set_rest_density", "pcisph_set_particle_radius", "pcisph_set_gravity",
+    "pcisph_set_viscosity", "pcisph_set_surface_tension",
+    "pcisph_set_density_variation_threshold", "pcisph_get_params",
+    "pcisph_set_solver_options", "pcisph_get_solver_options", "pcisph_set_precision",
+    "pcisph_check_buffers_consistent", "pcisph_field_size", "pcisph_upload",
+    "pcisph_download", "pcisph_upload_device", "pcisph_download_device",
+    "pcisph_download_f64", "pcisph_step", "pcisph_run", "pcisph_sync",
+    "pcisph_compute_params", "pcisph_profile_enable", "pcisph_profile_read",
+)
+
+STAGE_COUNT = 9
+
+
+class StageTimes(C.Structure):
+    _fields_ = [("ms", C.c_double * 16), ("launches", C.c_uint64 * 16)]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libpcisph_amd.so (built by pcisph_amd.build).  Raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m pcisph_amd.build`")
+    L = C.CDLL(path)
+    for name in EXPORTS:
+        getattr(L, name).restype = C.c_int
+    L.pcisph_last_error.restype = C.c_char_p
+    L.pcisph_last_error.argtypes = [C.c_void_p]
+    L.pcisph_create.argtypes = [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]
+    L.pcisph_destroy.argtypes = [C.c_void_p]
+    L.pcisph_compute_params.argtypes = [C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, C.c_void_p]
+    L.pcisph_run.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    L.pcisph_set_precision.argtypes = [C.c_void_p, C.c_int]
+    _lib = L
+    return L
+
+
+def compute_params(particle_radius, rest_density, boundary_count, fluid_count, viscosity,
+                   surface_tension, gravity, delta_t) -> SimulationParams:
+    """Derived parameters of a parameter set (host only, no device needed)."""
+    p = SimulationParams()
+    rc = load_library().pcisph_compute_params(particle_radius, rest_density, boundary_count,
+                                              fluid_count, viscosity, surface_tension, gravity,
+                                              delta_t, C.byref(p))
+    if rc:
+        raise PcisphError(rc, "pcisph_compute_params")
+    return p
+
+
+class Fluid(SolverBase):
+    """The gfx950 step behind the `sim::Fluid` interface.
+
+    precision: PRECISION_FAST (default), PRECISION_IEEE (bit-compatible FP32 order of
+    operations) or PRECISION_FP64."""
+
+    def __init__(self, device: int = 0, stream: int | None = None, precision: int = PRECISION_FAST):
+        self._lib = load_library()
+        self._prefix = "pcisph_"
+        h = C.c_void_p()
+        rc = self._lib.pcisph_create(int(device), C.c_void_p(stream) if stream else None, C.byref(h))
+        if rc:
+            msg = self._lib.pcisph_last_error(None).decode()
+            raise PcisphError(rc, f"pcisph_create: {msg}")
+        self._h = h
+        if precision != PRECISION_FAST:
+            self.set_precision(precision)
+
+    def _error_message(self) -> str:
+        return self._lib.pcisph_last_error(self._h).decode()
+
+    def set_precision(self, precision: int):
+        self._call("set_precision", C.c_int(int(precision)))
+        self.precision = precision
+
+    def run(self, steps: int, collect: bool = True) -> list[dict]:
+        """`steps` consecutive steps; per-step stats when `collect`."""
+        if collect:
+            arr = (StepStats * steps)()
+            self._call("run", C.c_uint32(steps), C.cast(arr, C.c_void_p))
+            return [s.as_dict() for s in arr]
+        self._call("run", C.c_uint32(steps), None)
+        return []
+
+    def sync(self):
+        self._call("sync")
+
+    def profile_enable(self, on: bool = True):
+        self._call("profile_enable", C.c_int(1 if on else 0))
+
+    def profile_read(self) -> dict:
+        t = StageTimes()
+        self._call("profile_read", C.byref(t))
+        return {"ms": list(t.ms)[:STAGE_COUNT], "launches": list(t.launches)[:STAGE_COUNT]}
+
+    def download_f64(self, field: int) -> np.ndarray:
+        n = self.field_size(field)
+        out = np.empty(n, dtype=np.float64)
+        self._call("download_f64", C.c_int(int(field)), out.ctypes.data_as(C.c_void_p), C.c_size_t(n))
+        return out
+
+    def upload_device(self, field: int, device_ptr: int, count: int):
+        self._call("upload_device", C.c_int(int(field)), C.c_void_p(device_ptr), C.c_size_t(count))
+
+    def download_device(self, field: int, device_ptr: int, count: int):
+        self._call("download_device", C.c_int(int(field)), C.c_void_p(device_ptr), C.c_size_t(count))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pcisph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["Fluid", "Field", "PcisphError", "SimulationParams", "SolverOptions", "StepStats",
+           "load_library", "compute_params", "field_dtype", "PRECISION_FAST", "PRECISION_IEEE",
+           "PRECISION_FP64", "EXPORTS"]

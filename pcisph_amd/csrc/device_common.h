@@ -1,0 +1,103 @@
+// device_common.h - grid hashing, parameter blocks and vector helpers shared by
+// every gfx950 kernel of the step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcisph {
+
+template <typename R>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+    using type = float4;
+};
+template <>
+struct Vec4Of<double> {
+    using type = double4;
+};
+template <typename R>
+using V4 = typename Vec4Of<R>::type;
+
+// Bucket count plus what `key % bucket_count` needs without a hardware divide:
+// a mask when the count is a power of two, else Lemire's direct-remainder constant
+// M = floor((2^64-1)/B)+1, exact for every 32-bit dividend ("Faster Remainder by
+// Direct Computation", Lemire, Kaser & Kurz 2019).
+struct HashGrid {
+    uint32_t bucket_count;
+    uint32_t pow2_mask;  // bucket_count-1 if a power of two, else 0
+    uint64_t fastmod_m;
+};
+
+inline HashGrid make_hash_grid(uint32_t B) {
+    HashGrid g;
+    g.bucket_count = B;
+    g.pow2_mask = (B & (B - 1)) == 0 ? B - 1 : 0;
+    g.fastmod_m = UINT64_MAX / B + 1;
+    return g;
+}
+
+__device__ __forceinline__ uint32_t bucket_mod(uint32_t v, const HashGrid& g) {
+    if (g.pow2_mask) return v & g.pow2_mask;
+    const uint64_t low = g.fastmod_m * (uint64_t)v;
+    return (uint32_t)__umul64hi(low, (uint64_t)g.bucket_count);
+}
+
+// get_hash_key (PCISPH/data/kernels/grid_utils.cl:12-28), 32-bit wrap arithmetic.
+__device__ __forceinline__ uint32_t hash_key(uint32_t x, uint32_t y, uint32_t z, const HashGrid& g) {
+    const uint32_t h = (x * 19349663u) ^ (y * 73856093u) ^ (z * 83492791u);
+    const uint32_t v = (h << 6) + (x & 63u) + 4u * (y & 63u) + 16u * (z & 63u);
+    return bucket_mod(v & 0x7FFFFFFFu, g);
+}
+
+// convert_int_sat_rtn (grid_utils.cl:9-11): floor, saturate, NaN -> 0.
+template <typename R>
+__device__ __forceinline__ int32_t sat_floor(R v) {
+    if (v != v) return 0;
+    const R f = floor(v);
+    if (f >= (R)2147483648.0) return INT32_MAX;
+    if (f < (R)-2147483648.0) return INT32_MIN;
+    return (int32_t)f;
+}
+
+__device__ __forceinline__ float div_rn(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ double div_rn(double a, double b) { return a / b; }
+
+// The cell of a position: IEEE division in every mode, so keys are bit-exact.
+template <typename R>
+__device__ __forceinline__ int3 cell_of(R x, R y, R z, R cell) {
+    return make_int3(sat_floor<R>(div_rn(x, cell)), sat_floor<R>(div_rn(y, cell)),
+                     sat_floor<R>(div_rn(z, cell)));
+}
+
+// The 27 buckets around cell c in the reference order: offset index
+// i = (dx+1) + 3(dy+1) + 9(dz+1), x fastest (grid_utils.cl:65-79).  The three
+// per-axis hash products are formed once and combined per bucket.
+template <typename F>
+__device__ __forceinline__ void for_each_bucket(int3 c, const HashGrid& g, F&& fn) {
+    uint32_t hx[3], hy[3], hz[3], lx[3], ly[3], lz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t x = (uint32_t)c.x + (uint32_t)(d - 1);
+        const uint32_t y = (uint32_t)c.y + (uint32_t)(d - 1);
+        const uint32_t z = (uint32_t)c.z + (uint32_t)(d - 1);
+        hx[d] = x * 19349663u;
+        hy[d] = y * 73856093u;
+        hz[d] = z * 83492791u;
+        lx[d] = x & 63u;
+        ly[d] = 4u * (y & 63u);
+        lz[d] = 16u * (z & 63u);
+    }
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const uint32_t v = ((hx[dx] ^ hy[dy] ^ hz[dz]) << 6) + lx[dx] + ly[dy] + lz[dz];
+                fn(bucket_mod(v & 0x7FFFFFFFu, g));
+            }
+}
+
+}  // namespace pcisph

@@ -1,0 +1,178 @@
+// radix_sort.hip - stable LSD radix sort of (cell-hash key, particle index) pairs
+// for gfx950.  Replaces the reference's clogs::Radixsort call
+// (PCISPH/src/sim/Fluid.cpp:84-87,153,193), whose contract is a stable key/value
+// sort (libs/clogs/include/clogs/radixsort.h:182-184).  Written from that contract
+// and the published reduce-then-scan radix sort (Merrill & Grimshaw 2011).
+//
+// Per digit pass, three launches:
+//   upsweep   : each tile of 2048 keys builds its digit histogram in LDS and writes
+//               it digit-major, hist[d * tiles + t]
+//   scan      : one workgroup turns hist into exclusive global offsets
+//   downsweep : each tile re-reads its keys, ranks them stably with wavefront
+//               ballots (the lanes sharing a digit are found with one ballot per
+//               digit bit; a lane's rank is the popcount of those below it), adds
+//               the per-wavefront prefix kept in LDS, and scatters key and value.
+// Digit width is a runtime choice of 4..8 bits.
+#include <hip/hip_runtime.h>
+
+#include "sph_launch.h"
+
+namespace pcisph {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kThreads * kItems;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxRadix = 256;
+
+__global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict__ keys, uint32_t n,
+                                                       uint32_t shift, uint32_t bits,
+                                                       uint32_t* __restrict__ hist, uint32_t tiles) {
+    __shared__ uint32_t s_hist[kMaxRadix];
+    const uint32_t radix = 1u << bits, mask = radix - 1;
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) s_hist[d] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kTile;
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const uint32_t e = base + r * kThreads + threadIdx.x;
+        if (e < n) atomicAdd(&s_hist[(keys[e] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d * tiles + blockIdx.x] = s_hist[d];
+}
+
+// Exclusive scan of `count` words by one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ data, uint32_t count) {
+    __shared__ uint32_t s_part[1024];
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per;
+    const uint32_t hi = min(count, lo + per);
+    uint32_t sum = 0;
+    for (uint32_t i = lo; i < hi; ++i) sum += data[i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the 1024 partial sums
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restrict__ kin,
+                                                         const uint32_t* __restrict__ vin,
+                                                         uint32_t* __restrict__ kout,
+                                                         uint32_t* __restrict__ vout, uint32_t n,
+                                                         uint32_t shift, uint32_t bits,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         uint32_t tiles) {
+    __shared__ uint32_t s_run[kMaxRadix];            // next free global slot per digit
+    __shared__ uint32_t s_wave[kWaves][kMaxRadix];   // per-wavefront count, then base
+    const uint32_t radix = 1u << bits, mask = radix - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
+        s_run[d] = offsets[d * tiles + blockIdx.x];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s_wave[w][d] = 0;
+    }
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * kTile;
+    for (int r = 0; r < kItems; ++r) {
+        const uint32_t e = base + r * kThreads + threadIdx.x;
+        const bool valid = e < n;
+        const uint32_t key = valid ? kin[e] : 0u;
+        const uint32_t val = valid ? vin[e] : 0u;
+        const uint32_t digit = (key >> shift) & mask;
+        // lanes of this wavefront holding the same digit
+        unsigned long long same = __ballot(valid);
+        for (uint32_t b = 0; b < bits; ++b) {
+            const bool bit = (digit >> b) & 1u;
+            const unsigned long long set = __ballot(bit);
+            same &= bit ? set : ~set;
+        }
+        const uint32_t rank = (uint32_t)__popcll(same & lt);
+        if (valid && rank == 0) s_wave[wave][digit] = (uint32_t)__popcll(same);
+        __syncthreads();
+        for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
+            uint32_t run = s_run[d];
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = s_wave[w][d];
+                s_wave[w][d] = run;
+                run += c;
+            }
+            s_run[d] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = s_wave[wave][digit] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+        for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) s_wave[w][d] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value) {
+    if (!n) return;
+    const uint32_t blocks = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, L.stream, p, n, value);
+}
+
+void fill_u64(const Launch& L, unsigned long long* p, size_t n, unsigned long long value) {
+    if (!n) return;
+    const uint32_t blocks = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, L.stream, p, n, value);
+}
+
+size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits) {
+    const size_t tiles = (n + kTile - 1) / kTile;
+    return (size_t(1) << digit_bits) * (tiles ? tiles : 1);
+}
+
+int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
+               uint32_t digit_bits, uint32_t* scratch) {
+    if (n <= 1 || key_bits == 0) return 0;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    int cur = 0;
+    for (uint32_t shift = 0; shift < key_bits; shift += digit_bits) {
+        const uint32_t bits = key_bits - shift < digit_bits ? key_bits - shift : digit_bits;
+        const uint32_t words = (1u << bits) * tiles;
+        hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], n, shift, bits,
+                           scratch, tiles);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, L.stream, scratch, words);
+        hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], vals[cur],
+                           keys[cur ^ 1], vals[cur ^ 1], n, shift, bits, scratch, tiles);
+        cur ^= 1;
+    }
+    return cur;
+}
+
+}  // namespace pcisph

@@ -1,0 +1,113 @@
+// sph_ieee.hip - IEEE instantiations of the step kernels (built with
+// -ffp-contract=off): FP32 in the reference's order of operations, and FP64.
+// Also the precision-only data kernels (hash, reorder, layout conversion).
+#include "sph_launch_impl.h"
+
+namespace pcisph {
+
+namespace {
+
+template <typename R>
+__global__ void k_unpack_vec3(const float* __restrict__ src, V4<R>* __restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V4<R> v;
+    v.x = (R)src[3 * (size_t)i];
+    v.y = (R)src[3 * (size_t)i + 1];
+    v.z = (R)src[3 * (size_t)i + 2];
+    v.w = 0;
+    dst[i] = v;
+}
+
+template <typename R, typename O>
+__global__ void k_pack_vec3(const V4<R>* __restrict__ src, O* __restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V4<R> v = src[i];
+    dst[3 * (size_t)i] = (O)v.x;
+    dst[3 * (size_t)i + 1] = (O)v.y;
+    dst[3 * (size_t)i + 2] = (O)v.z;
+}
+
+template <typename R>
+__global__ void k_unpack_scalar(const float* __restrict__ src, R* __restrict__ dst, uint32_t n,
+                                uint32_t stride, uint32_t offset) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[(size_t)i * stride + offset] = (R)src[i];
+}
+
+template <typename R, typename O>
+__global__ void k_pack_scalar(const R* __restrict__ src, O* __restrict__ dst, uint32_t n, uint32_t stride,
+                              uint32_t offset) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = (O)src[(size_t)i * stride + offset];
+}
+
+}  // namespace
+
+template <typename R>
+void DataLaunch<R>::hash(const Launch& L, const V4<R>* pos, uint32_t n, R cell, const HashGrid& g,
+                         uint32_t* keys, uint32_t* vals) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_hash<R>), dim3(grid_for(n, L.block)), dim3(L.block), 0, L.stream, pos, n, cell, g,
+                       keys, vals);
+}
+
+template <typename R>
+void DataLaunch<R>::reorder(const Launch& L, uint32_t n, const uint32_t* keys, const uint32_t* src,
+                            const V4<R>* pin, const V4<R>* vin, V4<R>* pout, V4<R>* vout, uint2* cells) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_reorder<R>), dim3(grid_for(n, L.block)), dim3(L.block), 0, L.stream, n, keys, src,
+                       pin, vin, pout, vout, cells);
+}
+
+template <typename R>
+void DataLaunch<R>::unpack_vec3(const Launch& L, const float* src, V4<R>* dst, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_unpack_vec3<R>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n);
+}
+
+template <typename R>
+void DataLaunch<R>::pack_vec3(const Launch& L, const V4<R>* src, float* dst, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_pack_vec3<R, float>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n);
+}
+
+template <typename R>
+void DataLaunch<R>::pack_vec3_f64(const Launch& L, const V4<R>* src, double* dst, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_pack_vec3<R, double>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n);
+}
+
+template <typename R>
+void DataLaunch<R>::unpack_scalar(const Launch& L, const float* src, R* dst, uint32_t n, uint32_t stride,
+                                  uint32_t offset) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_unpack_scalar<R>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n,
+                       stride, offset);
+}
+
+template <typename R>
+void DataLaunch<R>::pack_scalar(const Launch& L, const R* src, float* dst, uint32_t n, uint32_t stride,
+                                uint32_t offset) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_pack_scalar<R, float>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n,
+                       stride, offset);
+}
+
+template <typename R>
+void DataLaunch<R>::pack_scalar_f64(const Launch& L, const R* src, double* dst, uint32_t n, uint32_t stride,
+                                    uint32_t offset) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_pack_scalar<R, double>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst,
+                       n, stride, offset);
+}
+
+template struct DataLaunch<float>;
+template struct DataLaunch<double>;
+template struct SphLaunch<float, false>;
+template struct SphLaunch<double, false>;
+
+}  // namespace pcisph

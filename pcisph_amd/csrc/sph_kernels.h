@@ -1,0 +1,549 @@
+// sph_kernels.h - the PCISPH neighbour and stream kernels for gfx950.
+//
+// Included by two translation units that differ only in floating-point contract:
+//   sph_ieee.hip  (-ffp-contract=off): IEEE single/double, one rounding per
+//                 operation, correctly rounded `/` and sqrt, the reference's order of
+//                 operations and summation.  FP32 here reproduces the CPU oracle bit
+//                 for bit; FP64 is the double-precision reference run.
+//   sph_fast.hip  (-ffp-contract=fast): FP32 with fused multiply-adds and hardware
+//                 reciprocal / square root, the accuracy class OpenCL allows the
+//                 reference's `/`, distance() and fast_length().
+//
+// Layout in HBM (one array per field, structure of arrays, 16-byte elements so a
+// candidate costs one dwordx4 load):
+//   pos   float4 (x, y, z, 0)          sorted positions of this step
+//   xr    float4 (x, y, z, rho)        written by the density pass
+//   vel, nrm, fo, fp, xp  float4       velocity, normal, other force, pressure force,
+//                                      predicted position
+//   xf    float4 (x, y, z, p/rho^2)    written by the pressure pass for the force pass
+//   bpos  float4 (x, y, z, 0), bxf float4 (x, y, z, p_b/rho0^2) for the boundary
+// Every kernel is one work-item per particle in the sorted order, so the 64 lanes of
+// a wavefront walk neighbouring buckets together and share their loads.
+#pragma once
+
+#include "device_common.h"
+
+namespace pcisph {
+
+constexpr int kMaxIterationsCap = 256;
+
+// Step-constant scalars, each product formed once on the host in the same order as
+// the reference kernels write it (e.g. `density *= particle_mass * poly6_normalization`,
+// PCISPH/data/kernels/pcisph.cl:90).
+template <typename R>
+struct SphParams {
+    R h, h2, cell, rho0, m, dt, delta, threshold;
+    R density_scale;   // m * K6                       (pcisph.cl:90, :252)
+    R normal_scale;    // h * m * K6'                  (pcisph.cl:109)
+    R visc_scale;      // mu * m * Kv                  (pcisph.cl:160)
+    R coh_scale;       // -gamma * m * m * Kst          (pcisph.cl:162)
+    R curv_scale;      // -gamma * m                    (pcisph.cl:163)
+    R pf_scale;        // -Ks * m * m                   (pcisph.cl:297)
+    R st_term;         // h^6 / 64
+    R gravity_force;   // m * g                         (pcisph.cl:167)
+    R rho0_sq;         // rho0 * rho0                   (pcisph.cl:282-283)
+    HashGrid grid;
+    uint32_t n, nb;
+    uint32_t min_iterations, max_iterations;
+};
+
+// Per-step device record.  max_dv[k] holds the bits of the non-negative maximum
+// density variation of iteration k (integer order = float order for x >= 0).
+struct LoopState {
+    unsigned long long max_dv[kMaxIterationsCap];
+    unsigned long long fluid_candidates;
+    unsigned long long boundary_candidates;
+    unsigned int nonfinite;
+    unsigned int pad[3];
+};
+
+template <typename R>
+struct Bits;
+template <>
+struct Bits<float> {
+    __device__ __host__ static unsigned long long of(float v) {
+        union {
+            float f;
+            uint32_t u;
+        } x{v};
+        return x.u;
+    }
+    __device__ __host__ static float to(unsigned long long b) {
+        union {
+            uint32_t u;
+            float f;
+        } x{(uint32_t)b};
+        return x.f;
+    }
+};
+template <>
+struct Bits<double> {
+    __device__ __host__ static unsigned long long of(double v) {
+        union {
+            double f;
+            unsigned long long u;
+        } x{v};
+        return x.u;
+    }
+    __device__ __host__ static double to(unsigned long long b) {
+        union {
+            unsigned long long u;
+            double f;
+        } x{b};
+        return x.f;
+    }
+};
+
+// Termination of iteration k (PCISPH/src/sim/Fluid.cpp:283-306 with the minimum and
+// maximum iteration counts as options): tested from the min_iterations-th iteration
+// on, stop when max dv / rho0 < threshold, and always after max_iterations.
+template <typename R>
+__device__ __host__ inline bool iteration_is_last(const unsigned long long* max_dv, uint32_t k,
+                                                  uint32_t min_it, uint32_t max_it, R rho0, R thr) {
+    if (k + 1 >= max_it) return true;
+    if (k + 1 < min_it) return false;
+#ifdef __HIP_DEVICE_COMPILE__
+    return div_rn(Bits<R>::to(max_dv[k]), rho0) < thr;
+#else
+    return Bits<R>::to(max_dv[k]) / rho0 < thr;
+#endif
+}
+
+// Iteration k runs unless an earlier one ended the loop.  Iterations after the last
+// one find max_dv == 0 (cleared per step) in their predecessor, which reads as
+// converged for any positive threshold, so one look back is enough.
+template <typename R>
+__device__ inline bool iteration_skipped(const LoopState* st, uint32_t k, const SphParams<R>& P) {
+    if (k == 0) return false;
+    return iteration_is_last<R>(st->max_dv, k - 1, P.min_iterations, P.max_iterations, P.rho0,
+                                P.threshold);
+}
+
+// ------------------------------------------------------------- arithmetic modes
+template <typename R, bool FAST>
+struct Ops;
+template <>
+struct Ops<float, false> {
+    __device__ static float div(float a, float b) { return __fdiv_rn(a, b); }
+    __device__ static float sqrt(float a) { return __fsqrt_rn(a); }
+};
+template <>
+struct Ops<float, true> {
+    __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+};
+template <>
+struct Ops<double, false> {
+    __device__ static double div(double a, double b) { return a / b; }
+    __device__ static double sqrt(double a) { return ::sqrt(a); }
+};
+
+template <typename R>
+__device__ __forceinline__ R dist2(R ax, R ay, R az, R bx, R by, R bz, R& dx, R& dy, R& dz) {
+    dx = ax - bx;
+    dy = ay - by;
+    dz = az - bz;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// poly6 without normalisation (pcisph.cl:8-16)
+template <typename R>
+__device__ __forceinline__ R poly6(R r2, R h2) {
+    if (r2 > h2) return R(0);
+    const R dr = h2 - r2;
+    return dr * dr * dr;
+}
+
+// Akinci cohesion spline (pcisph.cl:48-61)
+template <typename R>
+__device__ __forceinline__ R surface_tension_kernel(R r, R h, R st_term) {
+    if (r <= R(0.5) * h) {
+        const R t1 = h - r;
+        const R t2 = (t1 * t1 * t1) * (r * r * r);
+        return R(2) * t2 - st_term;
+    } else if (r <= h) {
+        const R t1 = h - r;
+        return (t1 * t1 * t1) * (r * r * r);
+    }
+    return R(0);
+}
+
+template <typename R>
+__device__ __forceinline__ int3 cell_of4(const V4<R>& p, R cell) {
+    return cell_of<R>(p.x, p.y, p.z, cell);
+}
+
+template <typename R, typename F>
+__device__ __forceinline__ void for_each_candidate(const uint2* __restrict__ cells, int3 c,
+                                                   const HashGrid& g, F&& body) {
+    for_each_bucket(c, g, [&](uint32_t key) {
+        const uint2 r = cells[key];
+        for (uint32_t j = r.x; j < r.y; ++j) body(j);
+    });
+}
+
+template <typename T>
+__device__ __forceinline__ T block_reduce_sum(T v, T* lds) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += lds[w];
+    return s;
+}
+
+// ------------------------------------------------------------- kernels
+
+// initialize (PCISPH/data/kernels/sort_utils.cl:12-23): key = hash(cell(pos)), src = id
+template <typename R>
+__global__ void k_hash(const V4<R>* __restrict__ pos, uint32_t n, R cell, HashGrid g,
+                       uint32_t* __restrict__ keys, uint32_t* __restrict__ src) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V4<R> p = pos[i];
+    const int3 c = cell_of4<R>(p, cell);
+    keys[i] = hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, g);
+    src[i] = i;
+}
+
+// reorder_and_insert_{boundary,fluid}_offsets (sort_utils.cl:25-78): gather by the
+// permutation and record bucket start/end at key transitions.
+template <typename R>
+__global__ void k_reorder(uint32_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ src,
+                          const V4<R>* __restrict__ pin, const V4<R>* __restrict__ vin,
+                          V4<R>* __restrict__ pout, V4<R>* __restrict__ vout, uint2* __restrict__ cells) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const uint32_t s = src[d];
+    pout[d] = pin[s];
+    if (vin) vout[d] = vin[s];
+    const uint32_t k = keys[d];
+    if (d == 0 || k != keys[d - 1]) cells[k].x = d;
+    if (d == n - 1 || k != keys[d + 1]) cells[k].y = d + 1;
+}
+
+// update_density (pcisph.cl:65-92), plus per-particle candidate / neighbour counts.
+// Boundary particle pressures are reset here too (boundary_pressure_initialization,
+// pcisph.cl:114-119): threads n .. n+nb-1 handle the boundary.
+template <typename R, bool FAST>
+__global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+                          const uint2* __restrict__ fcells, const V4<R>* __restrict__ pos,
+                          V4<R>* __restrict__ xr, uint4* __restrict__ counts, R* __restrict__ bp,
+                          V4<R>* __restrict__ bxf, LoopState* __restrict__ st) {
+    __shared__ unsigned long long lds[16];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cf_sum = 0, cb_sum = 0;
+    if (i < P.n) {
+        const V4<R> self = pos[i];
+        const int3 c = cell_of4<R>(self, P.cell);
+        R rho = 0;
+        uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
+        if (P.nb)
+            for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
+                const V4<R> o = bpos[j];
+                R dx, dy, dz;
+                const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+                rho += poly6<R>(r2, P.h2);
+                ++cb;
+                nb += !(r2 > P.h2);
+            });
+        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+            const V4<R> o = pos[j];
+            R dx, dy, dz;
+            const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+            rho += poly6<R>(r2, P.h2);
+            ++cf;
+            nf += !(r2 > P.h2);
+        });
+        rho *= P.density_scale;
+        V4<R> out = self;
+        out.w = rho;
+        xr[i] = out;
+        if (counts) counts[i] = make_uint4(cf, cb, nf, nb);
+        cf_sum = cf;
+        cb_sum = cb;
+    } else if (i < P.n + P.nb) {
+        const uint32_t b = i - P.n;
+        bp[b] = 0;
+        V4<R> o = bpos[b];
+        o.w = 0;
+        bxf[b] = o;
+    }
+    const unsigned long long a = block_reduce_sum(cf_sum, lds);
+    __syncthreads();
+    const unsigned long long b = block_reduce_sum(cb_sum, lds + 8);
+    if (threadIdx.x == 0) {
+        atomicAdd(&st->fluid_candidates, a);
+        atomicAdd(&st->boundary_candidates, b);
+    }
+}
+
+// update_normal (pcisph.cl:94-112)
+template <typename R, bool FAST>
+__global__ void k_normal(SphParams<R> P, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+                         V4<R>* __restrict__ nrm) {
+    using O = Ops<R, FAST>;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const V4<R> self = xr[i];
+    const int3 c = cell_of4<R>(self, P.cell);
+    R nx = 0, ny = 0, nz = 0;
+    for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+        const V4<R> o = xr[j];
+        R dx, dy, dz;
+        const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+        if (r2 > P.h2) return;  // poly6_d1 is exactly zero there (pcisph.cl:20-21)
+        const R w = (r2 - P.h2) * (r2 - P.h2);
+        nx += O::div(w * dx, o.w);
+        ny += O::div(w * dy, o.w);
+        nz += O::div(w * dz, o.w);
+    });
+    V4<R> out;
+    out.x = nx * P.normal_scale;
+    out.y = ny * P.normal_scale;
+    out.z = nz * P.normal_scale;
+    out.w = 0;
+    nrm[i] = out;
+}
+
+// symplectic Euler of update_position_and_velocity (pcisph.cl:175-193)
+template <typename R, bool FAST>
+__device__ __forceinline__ void integrate(const SphParams<R>& P, const V4<R>& x, const V4<R>& v, R fx, R fy,
+                                          R fz, V4<R>& xo, V4<R>& vo) {
+    using O = Ops<R, FAST>;
+    const R ax = O::div(fx, P.m), ay = O::div(fy, P.m), az = O::div(fz, P.m);
+    vo.x = v.x + ax * P.dt;
+    vo.y = v.y + ay * P.dt;
+    vo.z = v.z + az * P.dt;
+    vo.w = 0;
+    xo.x = x.x + vo.x * P.dt;
+    xo.y = x.y + vo.y * P.dt;
+    xo.z = x.z + vo.z * P.dt;
+    xo.w = 0;
+}
+
+// force_initialization (pcisph.cl:121-173) fused with the first prediction of the
+// correction loop (pcisph.cl:175-193 with F_p = 0, Fluid.cpp:253-260).
+template <typename R, bool FAST>
+__global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+                             const V4<R>* __restrict__ vel, const V4<R>* __restrict__ nrm,
+                             V4<R>* __restrict__ fo, R* __restrict__ p, V4<R>* __restrict__ fp,
+                             V4<R>* __restrict__ xp) {
+    using O = Ops<R, FAST>;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const V4<R> self = xr[i];
+    const V4<R> sv = vel[i];
+    const V4<R> sn = nrm[i];
+    const int3 c = cell_of4<R>(self, P.cell);
+    R vx = 0, vy = 0, vz = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
+    for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+        const V4<R> o = xr[j];
+        R dx, dy, dz;
+        const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+        if (r2 > P.h2) return;  // every term below vanishes for dist >= h
+        const R dist = O::sqrt(r2);
+        const V4<R> ov = vel[j];
+        if (o.w > R(0.0001f)) {
+            const R k = O::div(dist > P.h ? R(0) : P.h - dist, o.w);
+            vx += (ov.x - sv.x) * k;
+            vy += (ov.y - sv.y) * k;
+            vz += (ov.z - sv.z) * k;
+        }
+        if (dist > R(0.0001f) && dist < P.h) {
+            const V4<R> on = nrm[j];
+            const R corr = O::div(R(2) * P.rho0, self.w + o.w);
+            const R stk = surface_tension_kernel<R>(dist, P.h, P.st_term);
+            const R cs = corr * stk;
+            cx += cs * O::div(dx, dist);
+            cy += cs * O::div(dy, dist);
+            cz += cs * O::div(dz, dist);
+            ux += corr * (sn.x - on.x);
+            uy += corr * (sn.y - on.y);
+            uz += corr * (sn.z - on.z);
+        }
+    });
+    const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
+    const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
+    const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
+    V4<R> f;
+    f.x = ox;
+    f.y = oy;
+    f.z = oz;
+    f.w = 0;
+    fo[i] = f;
+    p[i] = 0;
+    V4<R> z;
+    z.x = z.y = z.z = z.w = 0;
+    fp[i] = z;
+    V4<R> xo, vo;
+    integrate<R, FAST>(P, self, sv, ox + R(0), oy + R(0), oz + R(0), xo, vo);
+    xp[i] = xo;
+}
+
+// update_pressure (pcisph.cl:212-263): fluid mode for threads 0..n-1, boundary mode
+// for threads n..n+nb-1 (the two launches of Fluid.cpp:263-280 are independent).
+// Also publishes x + p/rho^2 for the force pass and the per-iteration max dv.
+template <typename R, bool FAST>
+__global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict__ st,
+                           const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+                           const R* __restrict__ binit, const uint2* __restrict__ fcells,
+                           const V4<R>* __restrict__ xr, const V4<R>* __restrict__ xp, R* __restrict__ dv,
+                           R* __restrict__ p, V4<R>* __restrict__ xf, R* __restrict__ bp,
+                           V4<R>* __restrict__ bxf) {
+    using O = Ops<R, FAST>;
+    __shared__ unsigned long long lds[16];
+    if (iteration_skipped<R>(st, iter, P)) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long my_max = 0;
+    if (i < P.n) {
+        const V4<R> self = xr[i];
+        const V4<R> pr = xp[i];
+        const int3 c = cell_of4<R>(self, P.cell);
+        R pd = 0;
+        if (P.nb)
+            for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
+                const V4<R> o = bpos[j];
+                R dx, dy, dz;
+                pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+            });
+        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+            const V4<R> o = xp[j];
+            R dx, dy, dz;
+            pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+        });
+        pd *= P.density_scale;
+        const R v = fmax(R(0), pd - P.rho0);
+        dv[i] = v;
+        R pi = p[i];
+        if (v != R(0)) {
+            pi += v * P.delta;
+            p[i] = pi;
+        }
+        V4<R> out = self;
+        out.w = O::div(pi, self.w * self.w);
+        xf[i] = out;
+        my_max = Bits<R>::of(v);
+    } else if (i < P.n + P.nb) {
+        const uint32_t b = i - P.n;
+        const V4<R> self = bpos[b];
+        const int3 c = cell_of4<R>(self, P.cell);
+        R pd = 0;
+        pd += binit[b];
+        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+            const V4<R> o = xp[j];
+            R dx, dy, dz;
+            pd += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+        });
+        pd *= P.density_scale;
+        const R v = fmax(R(0), pd - P.rho0);
+        if (v != R(0)) {
+            const R pb = bp[b] + v * P.delta;
+            bp[b] = pb;
+            V4<R> out = self;
+            out.w = O::div(pb, P.rho0_sq);
+            bxf[b] = out;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(my_max, off);
+        my_max = o > my_max ? o : my_max;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = my_max;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = lds[w] > m ? lds[w] : m;
+        if (m) atomicMax(&st->max_dv[iter], m);
+    }
+}
+
+// update_pressure_force (pcisph.cl:265-300) fused with what follows it: the next
+// prediction (pcisph.cl:175-193 into xp, Fluid.cpp:253-260) or, when this iteration
+// ends the loop, the final in-place integration (Fluid.cpp:309-316) into pos_out/vel_out.
+template <typename R, bool FAST>
+__global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __restrict__ st,
+                                 const uint2* __restrict__ bcells, const V4<R>* __restrict__ bxf,
+                                 const uint2* __restrict__ fcells, const V4<R>* __restrict__ xf,
+                                 const V4<R>* __restrict__ fo, const V4<R>* __restrict__ vel,
+                                 V4<R>* __restrict__ fp, V4<R>* __restrict__ xp, V4<R>* __restrict__ pos_out,
+                                 V4<R>* __restrict__ vel_out, unsigned long long* host_bits,
+                                 unsigned int* host_tag, unsigned long long* host_cand, unsigned int tag) {
+    using O = Ops<R, FAST>;
+    if (iteration_skipped<R>(st, iter, P)) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        // Publish this iteration's max dv to host-visible memory (the pressure pass
+        // that produced it has completed): the host decides how far to enqueue
+        // without waiting on the stream.  Value first, then the step tag.
+        host_bits[iter] = st->max_dv[iter];
+        host_cand[0] = st->fluid_candidates;
+        host_cand[1] = st->boundary_candidates;
+        __threadfence_system();
+        __hip_atomic_store(&host_tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    if (i >= P.n) return;
+    const bool last = iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations, P.rho0,
+                                           P.threshold);
+    const V4<R> self = xf[i];
+    const R sf = self.w;
+    const int3 c = cell_of4<R>(self, P.cell);
+    R fx = 0, fy = 0, fz = 0;
+    auto add = [&](const V4<R>& o) {
+        R dx, dy, dz;
+        const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+        if (r2 > P.h2) return;
+        const R r = O::sqrt(r2);
+        if (r > P.h || r < R(0.0001f)) return;  // kernel_spiky_d1 (pcisph.cl:28-37)
+        const R dr = P.h - r;
+        const R factor = sf + o.w;
+        fx += O::div(dx * dr * dr, r) * factor;
+        fy += O::div(dy * dr * dr, r) * factor;
+        fz += O::div(dz * dr * dr, r) * factor;
+    };
+    if (P.nb) for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) { add(bxf[j]); });
+    for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) { add(xf[j]); });
+    V4<R> f;
+    f.x = fx * P.pf_scale;
+    f.y = fy * P.pf_scale;
+    f.z = fz * P.pf_scale;
+    f.w = 0;
+    fp[i] = f;
+    const V4<R> o = fo[i];
+    const V4<R> v = vel[i];
+    V4<R> xo, vo;
+    integrate<R, FAST>(P, self, v, o.x + f.x, o.y + f.y, o.z + f.z, xo, vo);
+    if (!last) {
+        xp[i] = xo;
+    } else {
+        pos_out[i] = xo;
+        vel_out[i] = vo;
+        const bool bad = !(isfinite(xo.x) && isfinite(xo.y) && isfinite(xo.z) && isfinite(vo.x) &&
+                           isfinite(vo.y) && isfinite(vo.z));
+        if (bad) atomicOr(&st->nonfinite, 1u);
+    }
+}
+
+// initialize_boundary_boundary_pred_densities (pcisph.cl:195-210), opt-in.
+template <typename R, bool FAST>
+__global__ void k_boundary_init(SphParams<R> P, const uint2* __restrict__ bcells,
+                                const V4<R>* __restrict__ bpos, R* __restrict__ binit) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= P.nb) return;
+    const V4<R> self = bpos[b];
+    const int3 c = cell_of4<R>(self, P.cell);
+    R r = 0;
+    for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
+        const V4<R> o = bpos[j];
+        R dx, dy, dz;
+        r += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+    });
+    binit[b] = r;
+}
+
+}  // namespace pcisph

@@ -1,0 +1,94 @@
+// sph_launch.h - host-side launchers of the step's kernels.  The kernels are
+// instantiated in sph_ieee.hip (IEEE FP32 and FP64) and sph_fast.hip (fast FP32);
+// the solver calls them only through these declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_kernels.h"
+
+namespace pcisph {
+
+// Device arrays of one solver (precision R).  Fluid arrays hold N elements,
+// boundary arrays Nb, cell tables B (uint2 start/end).
+template <typename R>
+struct DeviceArrays {
+    uint32_t n = 0, nb = 0, B = 0;
+    // fluid: public state in the previous step's order, and this step's sorted copies
+    V4<R>* pos_in = nullptr;
+    V4<R>* vel_in = nullptr;
+    V4<R>* pos = nullptr;
+    V4<R>* vel = nullptr;
+    V4<R>* xr = nullptr;
+    V4<R>* nrm = nullptr;
+    V4<R>* fo = nullptr;
+    V4<R>* fp = nullptr;
+    V4<R>* xp = nullptr;
+    V4<R>* xf = nullptr;
+    R* p = nullptr;
+    R* dv = nullptr;
+    uint4* counts = nullptr;
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint2* cells = nullptr;
+    // boundary
+    V4<R>* bpos = nullptr;
+    V4<R>* bpos_tmp = nullptr;
+    V4<R>* bxf = nullptr;
+    R* bp = nullptr;
+    R* binit = nullptr;
+    uint32_t* bkeys[2] = {nullptr, nullptr};
+    uint32_t* bvals[2] = {nullptr, nullptr};
+    uint2* bcells = nullptr;
+    LoopState* state = nullptr;
+};
+
+struct Launch {
+    hipStream_t stream;
+    uint32_t block;
+};
+
+template <typename R, bool FAST>
+struct SphLaunch {
+    static void density(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
+    static void normal(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
+    static void force_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
+    static void pressure(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A, uint32_t iter);
+    static void pressure_force(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
+                               uint32_t iter, unsigned long long* host_bits, unsigned int* host_tag,
+                               unsigned long long* host_cand, unsigned int tag);
+    static void boundary_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
+};
+
+// Precision-only (no arithmetic mode) helpers, instantiated in sph_ieee.hip.
+template <typename R>
+struct DataLaunch {
+    static void hash(const Launch& L, const V4<R>* pos, uint32_t n, R cell, const HashGrid& g,
+                     uint32_t* keys, uint32_t* vals);
+    static void reorder(const Launch& L, uint32_t n, const uint32_t* keys, const uint32_t* src,
+                        const V4<R>* pin, const V4<R>* vin, V4<R>* pout, V4<R>* vout, uint2* cells);
+    // float[3n] <-> V4<R> (w set to 0 on the way in)
+    static void unpack_vec3(const Launch& L, const float* src, V4<R>* dst, uint32_t n);
+    static void pack_vec3(const Launch& L, const V4<R>* src, float* dst, uint32_t n);
+    static void pack_vec3_f64(const Launch& L, const V4<R>* src, double* dst, uint32_t n);
+    // float[n] <-> every `stride`-th R starting at `offset`
+    static void unpack_scalar(const Launch& L, const float* src, R* dst, uint32_t n, uint32_t stride,
+                              uint32_t offset);
+    static void pack_scalar(const Launch& L, const R* src, float* dst, uint32_t n, uint32_t stride,
+                            uint32_t offset);
+    static void pack_scalar_f64(const Launch& L, const R* src, double* dst, uint32_t n, uint32_t stride,
+                                uint32_t offset);
+};
+
+void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);
+void fill_u64(const Launch& L, unsigned long long* p, size_t n, unsigned long long value);
+
+// Stable LSD radix sort of (key, value) pairs on keys < 2^key_bits.  Sorts from
+// buffer 0 through buffer 1 alternately; returns the index (0/1) of the buffer that
+// holds the result.  `scratch` must hold radix_sort_scratch_words(n, digit_bits) words.
+size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits);
+int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
+               uint32_t digit_bits, uint32_t* scratch);
+
+}  // namespace pcisph
