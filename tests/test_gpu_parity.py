@@ -90,3 +90,19 @@ def test_iteration_bounds(gpu, oracle_lib):
         sg = run(g, scene, 2, min_iterations=mn, max_iterations=mx)
         assert [s["iterations"] for s in sg] == [s["iterations"] for s in so]
         assert all(mn <= s["iterations"] <= mx or s["iterations"] == mx for s in sg)
+
+
+@pytest.mark.parametrize("name,idx", [("small_dam_break_3steps", 0), ("dam_break_1000_2steps", 1)])
+def test_ieee_matches_golden(gpu, name, idx):
+    import os
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    scene = small_scenes()[idx]
+    print(f"case golden {name}", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    st = run(g, scene, len(ref["iterations"]), max_iterations=20)
+    assert [s["iterations"] for s in st] == list(ref["iterations"])
+    assert g.get_params().raw() == ref["params"].tobytes()
+    for k in ref.files:
+        if k in ("iterations", "max_density_variation", "params"):
+            continue
+        np.testing.assert_array_equal(g.download(Field[k]), ref[k], err_msg=k)
