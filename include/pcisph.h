@@ -176,6 +176,13 @@ int pcisph_step(pcisph_solver* s, pcisph_step_stats* stats);
 int pcisph_run(pcisph_solver* s, uint32_t n_steps, pcisph_step_stats* stats);
 int pcisph_sync(pcisph_solver* s);
 
+/* Apply pending setter changes now (derive parameters, reallocate, zero velocities,
+ * schedule the boundary sort) instead of at the next step, so that data uploaded
+ * afterwards -- e.g. the velocities of a resumed checkpoint -- survive the next
+ * step.  No counterpart in the reference, where this happens inside update()
+ * (Fluid.cpp:109-112). */
+int pcisph_prepare(pcisph_solver* s);
+
 /* Per-stage timing with HIP events on the solver's stream (the reference's
  * compiled-out step timer, Fluid.cpp:123,182,316,319-326, per stage).  Enabling
  * resets the totals; iterations enqueued ahead of the loop's end and then skipped

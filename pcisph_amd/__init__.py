@@ -32,7 +32,7 @@ EXPORTS = (
     "pcisph_check_buffers_consistent", "pcisph_field_size", "pcisph_upload",
     "pcisph_download", "pcisph_upload_device", "pcisph_download_device",
     "pcisph_download_f64", "pcisph_step", "pcisph_run", "pcisph_sync",
-    "pcisph_compute_params", "pcisph_profile_enable", "pcisph_profile_read",
+    "pcisph_compute_params", "pcisph_profile_enable", "pcisph_profile_read", "pcisph_prepare",
 )
 
 STAGE_COUNT = 9
@@ -114,6 +114,10 @@ class Fluid(SolverBase):
 
     def sync(self):
         self._call("sync")
+
+    def prepare(self):
+        """Apply pending setter changes now (see pcisph_prepare)."""
+        self._call("prepare")
 
     def profile_enable(self, on: bool = True):
         self._call("profile_enable", C.c_int(1 if on else 0))

@@ -16,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpcisph_amd.so")
+DRIVER = os.path.join(HERE, "pcisph_headless")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -57,8 +58,16 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # headless driver: C++ host program over the C ABI (rpath to this directory)
+    drv = os.path.join(HERE, "driver", "headless.cpp")
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", drv, "-o", DRIVER + ".tmp",
+           f"-L{HERE}", "-lpcisph_amd", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
 
 

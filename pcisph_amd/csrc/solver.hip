@@ -830,14 +830,17 @@ int pcisph_download_f64(pcisph_solver* s, int field, double* host, size_t count)
     return download_impl(s, field, host, count, false, true);
 }
 
+static void apply_param_changes(pcisph_solver* s) {
+    if (!s->params_changed) return;
+    derive_params(s->params);
+    s->eng->realloc_internal(s->params);
+    s->eng->zero_velocities();  // Fluid.cpp:462-463
+    s->boundary_updated = true;
+    s->params_changed = false;
+}
+
 static void step_impl(pcisph_solver* s, pcisph_step_stats* stats) {
-    if (s->params_changed) {
-        derive_params(s->params);
-        s->eng->realloc_internal(s->params);
-        s->eng->zero_velocities();  // Fluid.cpp:462-463
-        s->boundary_updated = true;
-        s->params_changed = false;
-    }
+    apply_param_changes(s);
     pcisph_step_stats st{};
     if (s->params.fluid_count) {
         const StepResult r = s->eng->step(s->params, s->threshold, s->opt,
@@ -868,6 +871,10 @@ int pcisph_run(pcisph_solver* s, uint32_t n_steps, pcisph_step_stats* stats) {
         for (uint32_t i = 0; i < n_steps; i++) step_impl(s, stats ? stats + i : nullptr);
         if (stats && n_steps && s->params.fluid_count) stats[n_steps - 1].nonfinite = s->eng->nonfinite();
     });
+}
+
+int pcisph_prepare(pcisph_solver* s) {
+    return guarded(s, [&] { apply_param_changes(s); });
 }
 
 int pcisph_sync(pcisph_solver* s) {
