@@ -99,9 +99,13 @@ typedef struct pcisph_step_stats {
     float max_density_variation;    /* max over fluid of max(0, rho* - rho0), last iteration */
     float max_density_error;        /* max_density_variation / rest_density (FP32, as tested) */
     uint32_t nonfinite;             /* 1 if a non-finite position/velocity was produced */
-    uint32_t reserved;
+    uint32_t warm_start;            /* bit 0: the network's start was evaluated; bit 1: it was
+                                       rejected for the plain start (larger residual) */
     uint64_t fluid_candidates;      /* sum over fluid particles of fluid candidates */
     uint64_t boundary_candidates;   /* sum over fluid particles of boundary candidates */
+    float max_density_expansion;    /* max over fluid of max(0, rho0 - rho*), last iteration:
+                                       the side of the residual the loop does not bound */
+    uint32_t reserved;
 } pcisph_step_stats;
 
 typedef struct pcisph_solver pcisph_solver;
@@ -197,7 +201,8 @@ typedef enum pcisph_stage {
     PCISPH_STAGE_FORCE_INIT = 6,    /* pcisph.cl:121-173 + first prediction */
     PCISPH_STAGE_PRESSURE = 7,      /* pcisph.cl:212-263, boundary and fluid */
     PCISPH_STAGE_PRESSURE_FORCE = 8,/* pcisph.cl:265-300 + prediction / final integration */
-    PCISPH_STAGE_COUNT = 9
+    PCISPH_STAGE_WARM_START = 9,    /* network inference + its pressure force */
+    PCISPH_STAGE_COUNT = 10
 } pcisph_stage;
 
 typedef struct pcisph_stage_times {
@@ -208,6 +213,35 @@ typedef struct pcisph_stage_times {
 int pcisph_profile_enable(pcisph_solver* s, int enable);
 /* Waits for the stream, then returns the totals since profiling was enabled. */
 int pcisph_profile_read(pcisph_solver* s, pcisph_stage_times* out);
+
+/* ---- learned warm start (no counterpart in the reference) ------------------------
+ * A network 8 -> 64 SiLU -> 64 SiLU -> 1 (BF16 weights and activations on the matrix
+ * cores, FP32 accumulation) predicts each fluid particle's converged pressure from
+ * local features (see PCISPH_MLP_INPUTS).  With the warm start on, the correction
+ * loop starts from that pressure instead of zero; its first iteration evaluates both
+ * starts and keeps the one with the smaller residual max |rho* - rho0|, so a bad
+ * prediction falls back to the plain loop, and termination is the unchanged test.
+ * Parameters: W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3 (FP32). */
+#define PCISPH_MLP_INPUTS 8
+#define PCISPH_MLP_PARAMS 4801
+int pcisph_mlp_set_params(pcisph_solver* s, const float* params, size_t count);
+int pcisph_mlp_get_params(pcisph_solver* s, float* params, size_t count);
+/* y[i] = net(x[i]) for n host rows x[n][8]. */
+int pcisph_mlp_forward(pcisph_solver* s, const float* x, float* y, size_t n);
+/* Gradient of 0.5 * sum (net(x) - t)^2 / n w.r.t. the parameters, and that loss. */
+int pcisph_mlp_gradients(pcisph_solver* s, const float* x, const float* t, size_t n, float* grad, float* loss);
+/* Device sample store: (features, converged pressure / p_ref) pairs. */
+int pcisph_samples_reserve(pcisph_solver* s, size_t capacity);  /* clears the store */
+/* Every following step appends the samples of every `stride`-th fluid particle
+ * (rotating the phase each step); 0 stops collecting. */
+int pcisph_samples_collect(pcisph_solver* s, uint32_t stride);
+int pcisph_samples_count(const pcisph_solver* s, size_t* count);
+int pcisph_samples_download(pcisph_solver* s, float* x, float* t, size_t count);
+int pcisph_samples_upload(pcisph_solver* s, const float* x, const float* t, size_t count);
+/* `steps` Adam steps on random batches of the store; mean loss of the last step. */
+int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float learning_rate, uint64_t seed,
+                     float* last_loss);
+int pcisph_set_warm_start(pcisph_solver* s, int enable);
 
 /* Derived parameters for a parameter set without a device (host only). */
 int pcisph_compute_params(float particle_radius, float rest_density, uint32_t boundary_count,

@@ -33,9 +33,15 @@ EXPORTS = (
     "pcisph_download", "pcisph_upload_device", "pcisph_download_device",
     "pcisph_download_f64", "pcisph_step", "pcisph_run", "pcisph_sync",
     "pcisph_compute_params", "pcisph_profile_enable", "pcisph_profile_read", "pcisph_prepare",
+    "pcisph_mlp_set_params", "pcisph_mlp_get_params", "pcisph_mlp_forward", "pcisph_mlp_gradients",
+    "pcisph_samples_reserve", "pcisph_samples_collect", "pcisph_samples_count",
+    "pcisph_samples_download", "pcisph_samples_upload", "pcisph_mlp_train", "pcisph_set_warm_start",
 )
 
-STAGE_COUNT = 9
+MLP_INPUTS = 8
+MLP_PARAMS = 4801
+
+STAGE_COUNT = 10
 
 
 class StageTimes(C.Structure):
@@ -62,6 +68,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                         C.c_float, C.c_float, C.c_float, C.c_void_p]
     L.pcisph_run.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
     L.pcisph_set_precision.argtypes = [C.c_void_p, C.c_int]
+    L.pcisph_mlp_train.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.c_void_p]
+    L.pcisph_samples_collect.argtypes = [C.c_void_p, C.c_uint32]
+    L.pcisph_set_warm_start.argtypes = [C.c_void_p, C.c_int]
     _lib = L
     return L
 
@@ -127,6 +136,66 @@ class Fluid(SolverBase):
         self._call("profile_read", C.byref(t))
         return {"ms": list(t.ms)[:STAGE_COUNT], "launches": list(t.launches)[:STAGE_COUNT]}
 
+    # --- learned warm start
+    def mlp_set_params(self, params: np.ndarray):
+        a = np.ascontiguousarray(params, np.float32).ravel()
+        self._call("mlp_set_params", a.ctypes.data_as(C.c_void_p), C.c_size_t(a.size))
+
+    def mlp_get_params(self) -> np.ndarray:
+        out = np.empty(MLP_PARAMS, np.float32)
+        self._call("mlp_get_params", out.ctypes.data_as(C.c_void_p), C.c_size_t(MLP_PARAMS))
+        return out
+
+    def mlp_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        y = np.empty(x.shape[0], np.float32)
+        self._call("mlp_forward", x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p),
+                   C.c_size_t(x.shape[0]))
+        return y
+
+    def mlp_gradients(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32).ravel()
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._call("mlp_gradients", x.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
+                   C.c_size_t(x.shape[0]), g.ctypes.data_as(C.c_void_p), C.byref(loss))
+        return g, loss.value
+
+    def samples_reserve(self, capacity: int):
+        self._call("samples_reserve", C.c_size_t(capacity))
+
+    def samples_collect(self, stride: int):
+        self._call("samples_collect", C.c_uint32(stride))
+
+    def samples_count(self) -> int:
+        n = C.c_size_t()
+        self._call("samples_count", C.byref(n))
+        return n.value
+
+    def samples_download(self):
+        n = self.samples_count()
+        x = np.empty((n, MLP_INPUTS), np.float32)
+        t = np.empty(n, np.float32)
+        self._call("samples_download", x.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
+                   C.c_size_t(n))
+        return x, t
+
+    def samples_upload(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32).ravel()
+        self._call("samples_upload", x.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
+                   C.c_size_t(x.shape[0]))
+
+    def mlp_train(self, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> float:
+        loss = C.c_float()
+        self._call("mlp_train", C.c_uint32(steps), C.c_uint32(batch), C.c_float(lr), C.c_uint64(seed),
+                   C.byref(loss))
+        return loss.value
+
+    def set_warm_start(self, on: bool = True):
+        self._call("set_warm_start", C.c_int(1 if on else 0))
+
     def download_f64(self, field: int) -> np.ndarray:
         n = self.field_size(field)
         out = np.empty(n, dtype=np.float64)
@@ -153,4 +222,4 @@ class Fluid(SolverBase):
 
 __all__ = ["Fluid", "Field", "PcisphError", "SimulationParams", "SolverOptions", "StepStats",
            "load_library", "compute_params", "field_dtype", "PRECISION_FAST", "PRECISION_IEEE",
-           "PRECISION_FP64", "EXPORTS"]
+           "PRECISION_FP64", "EXPORTS", "MLP_INPUTS", "MLP_PARAMS"]

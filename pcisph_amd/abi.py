@@ -62,9 +62,11 @@ class StepStats(C.Structure):
         ("max_density_variation", C.c_float),
         ("max_density_error", C.c_float),
         ("nonfinite", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("warm_start", C.c_uint32),
         ("fluid_candidates", C.c_uint64),
         ("boundary_candidates", C.c_uint64),
+        ("max_density_expansion", C.c_float),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

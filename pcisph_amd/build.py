@@ -32,6 +32,7 @@ UNITS = {
     "radix_sort.hip": ["-ffp-contract=off"],
     "sph_ieee.hip": ["-ffp-contract=off"],
     "sph_fast.hip": ["-ffp-contract=fast"],
+    "mlp.hip": ["-ffp-contract=fast"],
 }
 
 

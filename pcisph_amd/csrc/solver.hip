@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/pcisph.h"
+#include "mlp.h"
 #include "sph_launch.h"
 
 namespace pcisph {
@@ -217,8 +218,20 @@ class StageProfiler {
 struct StepResult {
     uint32_t iterations = 0;
     uint32_t converged = 0;
-    double max_dv = 0;
+    double max_dv = 0, max_expansion = 0;
     uint64_t fluid_candidates = 0, boundary_candidates = 0;
+    bool warm = false, warm_rejected = false;
+    uint64_t samples = 0;
+};
+
+// What a step does for the warm-start network: use it (params != null) and/or
+// append (features, converged pressure) samples of every stride-th particle.
+struct WarmArgs {
+    const float* params = nullptr;
+    uint32_t collect_stride = 0, collect_phase = 0;
+    float* store_x = nullptr;
+    float* store_t = nullptr;
+    uint64_t store_base = 0, store_cap = 0;
 };
 
 class EngineBase {
@@ -230,7 +243,7 @@ class EngineBase {
     virtual void upload(int field, const void* src, size_t count, bool device) = 0;
     virtual void download(int field, void* dst, size_t count, bool device, bool f64) = 0;
     virtual StepResult step(const Simulation_Params& p, float threshold, const pcisph_solver_options& o,
-                            bool sort_boundary, bool fast) = 0;
+                            bool sort_boundary, bool fast, const WarmArgs& w) = 0;
     virtual uint32_t nonfinite() = 0;
     virtual size_t field_size(int field, const Simulation_Params& p) const = 0;
     virtual void profile_enable(bool on) = 0;
@@ -281,18 +294,9 @@ class Engine final : public EngineBase {
    public:
     Engine(int device, hipStream_t stream) : device_(device), stream_(stream) {
         HIP_CHECK(hipSetDevice(device_));
-        HIP_CHECK(hipHostMalloc((void**)&host_bits_, sizeof(unsigned long long) * kMaxIterationsCap,
-                                hipHostMallocMapped | hipHostMallocCoherent));
-        HIP_CHECK(hipHostMalloc((void**)&host_tag_, sizeof(unsigned int) * kMaxIterationsCap,
-                                hipHostMallocMapped | hipHostMallocCoherent));
-        HIP_CHECK(hipHostMalloc((void**)&host_cand_, sizeof(unsigned long long) * 2,
-                                hipHostMallocMapped | hipHostMallocCoherent));
-        std::memset(host_bits_, 0, sizeof(unsigned long long) * kMaxIterationsCap);
-        std::memset((void*)host_tag_, 0, sizeof(unsigned int) * kMaxIterationsCap);
-        std::memset(host_cand_, 0, sizeof(unsigned long long) * 2);
-        HIP_CHECK(hipHostGetDevicePointer((void**)&dev_host_bits_, host_bits_, 0));
-        HIP_CHECK(hipHostGetDevicePointer((void**)&dev_host_tag_, (void*)host_tag_, 0));
-        HIP_CHECK(hipHostGetDevicePointer((void**)&dev_host_cand_, host_cand_, 0));
+        HIP_CHECK(hipHostMalloc((void**)&host_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset((void*)host_, 0, sizeof(HostMirror));
+        HIP_CHECK(hipHostGetDevicePointer((void**)&dev_host_, (void*)host_, 0));
         dalloc(A_.state, 1, stream_);
     }
 
@@ -302,9 +306,14 @@ class Engine final : public EngineBase {
         free_internal();
         dfree(A_.state);
         dfree(staging_);
-        if (host_bits_) (void)hipHostFree(host_bits_);
-        if (host_tag_) (void)hipHostFree((void*)host_tag_);
-        if (host_cand_) (void)hipHostFree(host_cand_);
+        dfree(feat_);
+        dfree(D_.xp0);
+        dfree(D_.xf0);
+        dfree(D_.p0);
+        dfree(D_.dv0);
+        dfree(D_.bp0);
+        dfree(D_.bxf0);
+        if (host_) (void)hipHostFree((void*)host_);
     }
 
     void free_public() {
@@ -481,13 +490,13 @@ class Engine final : public EngineBase {
     void wait_for_iteration(uint32_t k, unsigned int tag) {
         auto t0 = std::chrono::steady_clock::now();
         uint64_t spins = 0;
-        while (__atomic_load_n(&host_tag_[k], __ATOMIC_ACQUIRE) != tag) {
+        while (__atomic_load_n(&host_->tag[k], __ATOMIC_ACQUIRE) != tag) {
             _mm_pause();
             if ((++spins & 0x3FF) == 0) {
                 const hipError_t q = hipStreamQuery(stream_);
                 if (q != hipSuccess && q != hipErrorNotReady)
                     throw Error(PCISPH_ERR_DEVICE, std::string("stream failed: ") + hipGetErrorString(q));
-                if (q == hipSuccess && __atomic_load_n(&host_tag_[k], __ATOMIC_ACQUIRE) != tag)
+                if (q == hipSuccess && __atomic_load_n(&host_->tag[k], __ATOMIC_ACQUIRE) != tag)
                     throw Error(PCISPH_ERR_INTERNAL,
                                 "correction iteration " + std::to_string(k) + " finished without a result");
                 const double s =
@@ -498,13 +507,17 @@ class Engine final : public EngineBase {
     }
 
     StepResult step(const Simulation_Params& p, float threshold, const pcisph_solver_options& o,
-                    bool sort_boundary, bool fast) override {
+                    bool sort_boundary, bool fast, const WarmArgs& w) override {
         HIP_CHECK(hipSetDevice(device_));
         const Launch L{stream_, o.block_size ? o.block_size : 256u};
+        const MlpLaunch ML{stream_};
         const uint32_t digit = o.sort_digit_bits ? o.sort_digit_bits : 8u;
         const SphParams<R> P = make_sph_params<R>(p, threshold, o);
         const uint32_t key_bits = key_bits_for(p.bucket_count);
         const unsigned int tag = ++tag_ ? tag_ : ++tag_;
+        const bool warm = w.params != nullptr;
+        const bool features = warm || w.collect_stride;
+        if (features) ensure_warm_buffers(warm);
         fill_u64(L, (unsigned long long*)A_.state, sizeof(LoopState) / 8, 0ull);
         if (sort_boundary && A_.nb) {
             prof_.timed(PCISPH_STAGE_BOUNDARY_SORT, stream_, -1, [&] {
@@ -531,7 +544,19 @@ class Engine final : public EngineBase {
         const LaunchTable ops = launch(fast);
         prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, P, A_); });
         prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, P, A_); });
-        prof_.timed(PCISPH_STAGE_FORCE_INIT, stream_, -1, [&] { ops.force_init(L, P, A_); });
+        const float p_ref = (float)(P.delta * P.rho0 / R(100));
+        FeatureArgs F{features ? feat_ : nullptr, A_.counts, p_ref,
+                      (float)((double)p.particle_mass * 45.0 / (3.14159265358979323846 * std::pow((double)P.h, 6)) *
+                              (double)P.dt * 100.0)};
+        prof_.timed(PCISPH_STAGE_FORCE_INIT, stream_, -1, [&] { ops.force_init(L, P, A_, F); });
+        if (warm) {
+            // network start: p from the features, its pressure force, and a prediction
+            // with it; the plain prediction is kept for the dual first iteration
+            prof_.timed(PCISPH_STAGE_WARM_START, stream_, -1, [&] {
+                mlp_warm_start<R, V4<R>>(ML, w.params, feat_, A_.n, p_ref, A_.xr, A_.p, A_.xf);
+                ops.pressure_force(L, P, A_, 0, dev_host_, tag, 1, D_.xp0);
+            });
+        }
         HIP_CHECK(hipGetLastError());
 
         const uint32_t maxit = o.max_iterations, minit = std::max(o.min_iterations, 1u);
@@ -539,9 +564,12 @@ class Engine final : public EngineBase {
         const uint32_t lookahead = 1;
         uint32_t enq = 0;
         auto enqueue = [&]() {
-            prof_.timed(PCISPH_STAGE_PRESSURE, stream_, (int)enq, [&] { ops.pressure(L, P, A_, enq); });
+            prof_.timed(PCISPH_STAGE_PRESSURE, stream_, (int)enq, [&] {
+                if (warm && enq == 0) ops.pressure_dual(L, P, A_, D_);
+                else ops.pressure(L, P, A_, enq);
+            });
             prof_.timed(PCISPH_STAGE_PRESSURE_FORCE, stream_, (int)enq, [&] {
-                ops.pressure_force(L, P, A_, enq, dev_host_bits_, dev_host_tag_, dev_host_cand_, tag);
+                ops.pressure_force(L, P, A_, enq, dev_host_, tag, 0, nullptr);
             });
             ++enq;
         };
@@ -550,49 +578,78 @@ class Engine final : public EngineBase {
         uint32_t k = first;
         for (;;) {
             wait_for_iteration(k, tag);
-            if (iteration_is_last<R>(host_bits_, k, minit, maxit, P.rho0, P.threshold)) break;
+            if (iteration_is_last<R>((const unsigned long long*)host_->max_dv, k, minit, maxit, P.rho0,
+                                     P.threshold))
+                break;
             ++k;
             if (enq < maxit) enqueue();
         }
         HIP_CHECK(hipGetLastError());
+        StepResult r;
+        if (w.collect_stride) {
+            const uint32_t phase = w.collect_phase % w.collect_stride;
+            mlp_collect<R>(ML, feat_, A_.p, A_.n, w.collect_stride, phase, p_ref, w.store_x, w.store_t,
+                           w.store_base, w.store_cap);
+            r.samples = A_.n > phase ? (A_.n - phase + w.collect_stride - 1) / w.collect_stride : 0;
+        }
         prof_.end_step((int)k);
         prof_.maybe_resolve(stream_);
-        StepResult r;
         r.iterations = k + 1;
-        const R mdv = Bits<R>::to(host_bits_[k]);
+        const R mdv = Bits<R>::to(host_->max_dv[k]);
         r.max_dv = (double)mdv;
+        r.max_expansion = (double)Bits<R>::to(host_->max_exp[k]);
         r.converged = (k + 1 >= minit) && (mdv / P.rho0 < P.threshold);
-        r.fluid_candidates = host_cand_[0];
-        r.boundary_candidates = host_cand_[1];
+        r.fluid_candidates = host_->candidates[0];
+        r.boundary_candidates = host_->candidates[1];
+        r.warm = warm;
+        r.warm_rejected = warm && host_->warm_rejected;
         return r;
+    }
+
+    void ensure_warm_buffers(bool dual) {
+        const uint32_t n = A_.n, nb = A_.nb;
+        if (feat_n_ != n) {
+            dalloc(feat_, 8 * (size_t)std::max(n, 1u), stream_);
+            feat_n_ = n;
+        }
+        if (dual && (dual_n_ != n || dual_nb_ != nb)) {
+            dalloc(D_.xp0, n, stream_);
+            dalloc(D_.xf0, n, stream_);
+            dalloc(D_.p0, n, stream_);
+            dalloc(D_.dv0, n, stream_);
+            dalloc(D_.bp0, std::max(nb, 1u), stream_);
+            dalloc(D_.bxf0, std::max(nb, 1u), stream_);
+            dual_n_ = n;
+            dual_nb_ = nb;
+        }
     }
 
    private:
     struct LaunchTable {
         void (*density)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&);
         void (*normal)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&);
-        void (*force_init)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&);
+        void (*force_init)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, const FeatureArgs&);
         void (*pressure)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, uint32_t);
-        void (*pressure_force)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, uint32_t,
-                               unsigned long long*, unsigned int*, unsigned long long*, unsigned int);
+        void (*pressure_force)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, uint32_t, HostMirror*,
+                               unsigned int, int, V4<R>*);
         void (*boundary_init)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&);
+        void (*pressure_dual)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, const DualArrays<R>&);
     };
     static LaunchTable launch(bool fast);
 
     int device_;
     hipStream_t stream_;
     DeviceArrays<R> A_;
+    DualArrays<R> D_{};
     int fluid_sorted_ = 0, boundary_sorted_ = 0;
     uint32_t* sort_scratch_ = nullptr;
     size_t sort_scratch_words_ = 0;
     float* staging_ = nullptr;
     size_t staging_words_ = 0;
-    unsigned long long* host_bits_ = nullptr;
-    volatile unsigned int* host_tag_ = nullptr;
-    unsigned long long* host_cand_ = nullptr;
-    unsigned long long* dev_host_bits_ = nullptr;
-    unsigned int* dev_host_tag_ = nullptr;
-    unsigned long long* dev_host_cand_ = nullptr;
+    float* feat_ = nullptr;
+    uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0;
+    volatile HostMirror* host_ = nullptr;
+    HostMirror* dev_host_ = nullptr;
     unsigned int tag_ = 0;
 
    public:
@@ -604,17 +661,18 @@ Engine<float>::LaunchTable Engine<float>::launch(bool fast) {
     if (fast)
         return {SphLaunch<float, true>::density, SphLaunch<float, true>::normal, SphLaunch<float, true>::force_init,
                 SphLaunch<float, true>::pressure, SphLaunch<float, true>::pressure_force,
-                SphLaunch<float, true>::boundary_init};
+                SphLaunch<float, true>::boundary_init, SphLaunch<float, true>::pressure_dual};
     return {SphLaunch<float, false>::density, SphLaunch<float, false>::normal, SphLaunch<float, false>::force_init,
             SphLaunch<float, false>::pressure, SphLaunch<float, false>::pressure_force,
-            SphLaunch<float, false>::boundary_init};
+            SphLaunch<float, false>::boundary_init, SphLaunch<float, false>::pressure_dual};
 }
 
 template <>
 Engine<double>::LaunchTable Engine<double>::launch(bool) {
     return {SphLaunch<double, false>::density, SphLaunch<double, false>::normal,
             SphLaunch<double, false>::force_init, SphLaunch<double, false>::pressure,
-            SphLaunch<double, false>::pressure_force, SphLaunch<double, false>::boundary_init};
+            SphLaunch<double, false>::pressure_force, SphLaunch<double, false>::boundary_init,
+            SphLaunch<double, false>::pressure_dual};
 }
 
 }  // namespace pcisph
@@ -636,6 +694,34 @@ struct pcisph_solver {
     int precision = kPrecisionFast;
     std::unique_ptr<EngineBase> eng;
     std::string err;
+    // warm-start network and its training data (device memory, allocated on first use)
+    float* mlp = nullptr;        // FP32 parameters
+    float* mlp_m = nullptr;      // Adam moments
+    float* mlp_v = nullptr;
+    float* mlp_grad = nullptr;
+    float* mlp_loss = nullptr;
+    uint32_t adam_step = 0;
+    bool warm_start = false;
+    float* samples_x = nullptr;  // [cap][8]
+    float* samples_t = nullptr;  // [cap]
+    uint64_t samples_cap = 0, samples_count = 0;
+    uint32_t collect_stride = 0, collect_phase = 0;
+
+    ~pcisph_solver() {
+        for (float* q : {mlp, mlp_m, mlp_v, mlp_grad, mlp_loss, samples_x, samples_t})
+            if (q) (void)hipFree(q);
+    }
+
+    void ensure_mlp() {
+        if (mlp) return;
+        HIP_CHECK(hipSetDevice(device));
+        for (float** q : {&mlp, &mlp_m, &mlp_v, &mlp_grad}) {
+            HIP_CHECK(hipMalloc((void**)q, sizeof(float) * kMlpParams));
+            HIP_CHECK(hipMemsetAsync(*q, 0, sizeof(float) * kMlpParams, stream));
+        }
+        HIP_CHECK(hipMalloc((void**)&mlp_loss, sizeof(float) * 4));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
 };
 
 static thread_local std::string g_create_error;
@@ -843,9 +929,22 @@ static void step_impl(pcisph_solver* s, pcisph_step_stats* stats) {
     apply_param_changes(s);
     pcisph_step_stats st{};
     if (s->params.fluid_count) {
+        WarmArgs w;
+        if (s->warm_start && s->mlp) w.params = s->mlp;
+        if (s->collect_stride && s->samples_x && s->samples_count < s->samples_cap) {
+            w.collect_stride = s->collect_stride;
+            w.collect_phase = s->collect_phase++;
+            w.store_x = s->samples_x;
+            w.store_t = s->samples_t;
+            w.store_base = s->samples_count;
+            w.store_cap = s->samples_cap;
+        }
         const StepResult r = s->eng->step(s->params, s->threshold, s->opt,
                                           s->boundary_updated || s->opt.resort_boundary,
-                                          s->precision == kPrecisionFast);
+                                          s->precision == kPrecisionFast, w);
+        s->samples_count = std::min(s->samples_cap, s->samples_count + r.samples);
+        st.warm_start = (r.warm ? 1u : 0u) | (r.warm_rejected ? 2u : 0u);
+        st.max_density_expansion = (float)r.max_expansion;
         s->boundary_updated = false;
         st.iterations = r.iterations;
         st.converged = r.converged;
@@ -879,6 +978,147 @@ int pcisph_prepare(pcisph_solver* s) {
 
 int pcisph_sync(pcisph_solver* s) {
     return guarded(s, [&] { HIP_CHECK(hipStreamSynchronize(s->stream)); });
+}
+
+// ------------------------------------------------------------------ warm start
+struct DeviceTemp {
+    void* p = nullptr;
+    explicit DeviceTemp(size_t bytes) { HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16))); }
+    ~DeviceTemp() { (void)hipFree(p); }
+    float* f() const { return (float*)p; }
+};
+
+int pcisph_mlp_set_params(pcisph_solver* s, const float* params, size_t count) {
+    return guarded(s, [&] {
+        if (!params || count != (size_t)kMlpParams)
+            throw Error(PCISPH_ERR_SIZE, "expected " + std::to_string(kMlpParams) + " parameters");
+        s->ensure_mlp();
+        HIP_CHECK(hipMemcpyAsync(s->mlp, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s->stream));
+        HIP_CHECK(hipMemsetAsync(s->mlp_m, 0, sizeof(float) * kMlpParams, s->stream));
+        HIP_CHECK(hipMemsetAsync(s->mlp_v, 0, sizeof(float) * kMlpParams, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        s->adam_step = 0;
+    });
+}
+
+int pcisph_mlp_get_params(pcisph_solver* s, float* params, size_t count) {
+    return guarded(s, [&] {
+        if (!params || count != (size_t)kMlpParams)
+            throw Error(PCISPH_ERR_SIZE, "expected " + std::to_string(kMlpParams) + " parameters");
+        s->ensure_mlp();
+        HIP_CHECK(hipMemcpyAsync(params, s->mlp, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pcisph_mlp_forward(pcisph_solver* s, const float* x, float* y, size_t n) {
+    return guarded(s, [&] {
+        if ((!x || !y) && n) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "null pointer");
+        s->ensure_mlp();
+        DeviceTemp dx(n * kMlpIn * sizeof(float)), dy(n * sizeof(float));
+        HIP_CHECK(hipMemcpyAsync(dx.p, x, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        mlp_forward(MlpLaunch{s->stream}, s->mlp, dx.f(), dy.f(), (uint32_t)n);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(y, dy.p, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pcisph_mlp_gradients(pcisph_solver* s, const float* x, const float* t, size_t n, float* grad, float* loss) {
+    return guarded(s, [&] {
+        if (!x || !t || !grad || !loss || !n) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "null pointer or empty batch");
+        s->ensure_mlp();
+        DeviceTemp dx(n * kMlpIn * sizeof(float)), dt(n * sizeof(float));
+        HIP_CHECK(hipMemcpyAsync(dx.p, x, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        HIP_CHECK(hipMemcpyAsync(dt.p, t, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        mlp_gradients(MlpLaunch{s->stream}, s->mlp, dx.f(), dt.f(), (uint32_t)n, s->mlp_grad, s->mlp_loss);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(grad, s->mlp_grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipMemcpyAsync(loss, s->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pcisph_samples_reserve(pcisph_solver* s, size_t capacity) {
+    return guarded(s, [&] {
+        HIP_CHECK(hipSetDevice(s->device));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        for (float** q : {&s->samples_x, &s->samples_t})
+            if (*q) {
+                (void)hipFree(*q);
+                *q = nullptr;
+            }
+        s->samples_cap = s->samples_count = 0;
+        if (!capacity) return;
+        HIP_CHECK(hipMalloc((void**)&s->samples_x, capacity * kMlpIn * sizeof(float)));
+        HIP_CHECK(hipMalloc((void**)&s->samples_t, capacity * sizeof(float)));
+        s->samples_cap = capacity;
+    });
+}
+
+int pcisph_samples_collect(pcisph_solver* s, uint32_t stride) {
+    return guarded(s, [&] {
+        if (stride && !s->samples_x) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "reserve a sample store first");
+        if (stride && s->precision == kPrecisionFp64)
+            throw Error(PCISPH_ERR_INVALID_ARGUMENT, "samples are collected from FP32 solvers");
+        s->collect_stride = stride;
+    });
+}
+
+int pcisph_samples_count(const pcisph_solver* s, size_t* count) {
+    if (!s || !count) return PCISPH_ERR_INVALID_ARGUMENT;
+    *count = (size_t)s->samples_count;
+    return PCISPH_OK;
+}
+
+int pcisph_samples_download(pcisph_solver* s, float* x, float* t, size_t count) {
+    return guarded(s, [&] {
+        if (count > s->samples_count) throw Error(PCISPH_ERR_SIZE, "more samples requested than stored");
+        if (!count) return;
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        HIP_CHECK(hipMemcpy(x, s->samples_x, count * kMlpIn * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(t, s->samples_t, count * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int pcisph_samples_upload(pcisph_solver* s, const float* x, const float* t, size_t count) {
+    return guarded(s, [&] {
+        if (count > s->samples_cap) throw Error(PCISPH_ERR_SIZE, "sample store too small");
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        if (count) {
+            HIP_CHECK(hipMemcpy(s->samples_x, x, count * kMlpIn * sizeof(float), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(s->samples_t, t, count * sizeof(float), hipMemcpyHostToDevice));
+        }
+        s->samples_count = count;
+    });
+}
+
+int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* last_loss) {
+    return guarded(s, [&] {
+        if (!s->samples_count) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "no samples to train on");
+        if (!batch) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "batch must be positive");
+        s->ensure_mlp();
+        const MlpLaunch ML{s->stream};
+        DeviceTemp bx((size_t)batch * kMlpIn * sizeof(float)), bt((size_t)batch * sizeof(float));
+        for (uint32_t i = 0; i < steps; ++i) {
+            mlp_gather_batch(ML, s->samples_x, s->samples_t, s->samples_count, seed, s->adam_step, batch, bx.f(),
+                             bt.f());
+            mlp_gradients(ML, s->mlp, bx.f(), bt.f(), batch, s->mlp_grad, s->mlp_loss);
+            mlp_adam(ML, s->mlp, s->mlp_grad, s->mlp_m, s->mlp_v, ++s->adam_step, lr, 0.9f, 0.999f, 1e-8f);
+        }
+        HIP_CHECK(hipGetLastError());
+        float loss = 0.f;
+        HIP_CHECK(hipMemcpyAsync(&loss, s->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        if (last_loss) *last_loss = loss / (float)batch;
+    });
+}
+
+int pcisph_set_warm_start(pcisph_solver* s, int enable) {
+    return guarded(s, [&] {
+        if (enable) s->ensure_mlp();
+        s->warm_start = enable != 0;
+    });
 }
 
 int pcisph_profile_enable(pcisph_solver* s, int enable) {

@@ -51,10 +51,50 @@ struct SphParams {
 // density variation of iteration k (integer order = float order for x >= 0).
 struct LoopState {
     unsigned long long max_dv[kMaxIterationsCap];
+    unsigned long long max_exp[kMaxIterationsCap];  // max(0, rho0 - rho*): the signed residual's other side
     unsigned long long fluid_candidates;
     unsigned long long boundary_candidates;
+    // iteration 0 of a warm-started step evaluates both starts: [0] warm, [1] plain
+    unsigned long long dual_dv[2];
+    unsigned long long dual_exp[2];
     unsigned int nonfinite;
-    unsigned int pad[3];
+    unsigned int warm_rejected;  // 1: the plain start had the smaller residual
+    unsigned int pad[2];
+};
+
+// Second set of per-particle arrays for the dual first iteration of a warm start:
+// the plain start's predicted positions, pressures, p/rho^2 and density variations.
+template <typename R>
+struct DualArrays {
+    V4<R>* xp0;
+    V4<R>* xf0;
+    R* p0;
+    R* dv0;
+    R* bp0;
+    V4<R>* bxf0;
+};
+
+// Host-visible copy of the loop's per-iteration results (pinned, mapped memory),
+// written by the first thread of each pressure-force launch.
+struct HostMirror {
+    unsigned long long max_dv[kMaxIterationsCap];
+    unsigned long long max_exp[kMaxIterationsCap];
+    unsigned long long candidates[2];
+    unsigned int warm_rejected;
+    unsigned int tag[kMaxIterationsCap];
+};
+
+// Inputs of the warm-start network, per fluid particle (8 floats, row-major):
+//  0 (rho/rho0 - 1) * 100          1 fluid neighbours / 32
+//  2 previous-step pressure / p_ref 3 |v| dt / h * 100
+//  4 neighbour mean of input 0      5 neighbour mean of input 2
+//  6 boundary neighbours / 32       7 velocity divergence * dt * 100
+// with p_ref = delta * rho0 / 100, the pressure one iteration assigns to a 1% error.
+struct FeatureArgs {
+    float* feat;          // float[n][8] or null (no features)
+    const uint4* counts;  // per-particle neighbour counts of the density pass
+    float p_ref;
+    float div_scale;      // m * 45 / (pi h^6) * dt * 100
 };
 
 template <typename R>
@@ -327,11 +367,13 @@ __device__ __forceinline__ void integrate(const SphParams<R>& P, const V4<R>& x,
 
 // force_initialization (pcisph.cl:121-173) fused with the first prediction of the
 // correction loop (pcisph.cl:175-193 with F_p = 0, Fluid.cpp:253-260).
+// With features requested it also gathers the warm-start network's inputs; the
+// velocity's w component carries the previous step's converged pressure.
 template <typename R, bool FAST>
 __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                              const V4<R>* __restrict__ vel, const V4<R>* __restrict__ nrm,
                              V4<R>* __restrict__ fo, R* __restrict__ p, V4<R>* __restrict__ fp,
-                             V4<R>* __restrict__ xp) {
+                             V4<R>* __restrict__ xp, FeatureArgs F) {
     using O = Ops<R, FAST>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
@@ -340,6 +382,8 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
     const V4<R> sn = nrm[i];
     const int3 c = cell_of4<R>(self, P.cell);
     R vx = 0, vy = 0, vz = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
+    float a_rho = 0.f, a_p = 0.f, a_div = 0.f, a_n = 0.f;
+    const bool want = F.feat != nullptr;
     for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
         const V4<R> o = xr[j];
         R dx, dy, dz;
@@ -347,6 +391,16 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
         if (r2 > P.h2) return;  // every term below vanishes for dist >= h
         const R dist = O::sqrt(r2);
         const V4<R> ov = vel[j];
+        if (want) {
+            a_n += 1.f;
+            a_rho += (float)o.w;
+            a_p += (float)ov.w;
+            if (dist > R(0.0001f)) {
+                const float dr = (float)(P.h - dist);
+                a_div += (float)(((ov.x - sv.x) * dx + (ov.y - sv.y) * dy + (ov.z - sv.z) * dz) / dist) * dr * dr /
+                         (float)o.w;
+            }
+        }
         if (o.w > R(0.0001f)) {
             const R k = O::div(dist > P.h ? R(0) : P.h - dist, o.w);
             vx += (ov.x - sv.x) * k;
@@ -382,6 +436,17 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
     V4<R> xo, vo;
     integrate<R, FAST>(P, self, sv, ox + R(0), oy + R(0), oz + R(0), xo, vo);
     xp[i] = xo;
+    if (want) {
+        const float rho0 = (float)P.rho0;
+        const uint4 cnt = F.counts[i];
+        const float inv = a_n > 0.f ? 1.f / a_n : 0.f;
+        const float speed = sqrtf((float)(sv.x * sv.x + sv.y * sv.y + sv.z * sv.z));
+        float4* out = reinterpret_cast<float4*>(F.feat + (size_t)i * 8);
+        out[0] = make_float4(((float)self.w / rho0 - 1.f) * 100.f, (float)cnt.z / 32.f, (float)sv.w / F.p_ref,
+                             speed * (float)P.dt / (float)P.h * 100.f);
+        out[1] = make_float4((a_rho * inv / rho0 - 1.f) * 100.f, a_p * inv / F.p_ref, (float)cnt.w / 32.f,
+                             a_div * F.div_scale);
+    }
 }
 
 // update_pressure (pcisph.cl:212-263): fluid mode for threads 0..n-1, boundary mode
@@ -398,7 +463,7 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
     __shared__ unsigned long long lds[16];
     if (iteration_skipped<R>(st, iter, P)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long my_max = 0;
+    unsigned long long my_max = 0, my_exp = 0;
     if (i < P.n) {
         const V4<R> self = xr[i];
         const V4<R> pr = xp[i];
@@ -417,6 +482,7 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
         });
         pd *= P.density_scale;
         const R v = fmax(R(0), pd - P.rho0);
+        my_exp = Bits<R>::of(fmax(R(0), P.rho0 - pd));
         dv[i] = v;
         R pi = p[i];
         if (v != R(0)) {
@@ -452,14 +518,23 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(my_max, off);
         my_max = o > my_max ? o : my_max;
+        const unsigned long long e = __shfl_xor(my_exp, off);
+        my_exp = e > my_exp ? e : my_exp;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) lds[wave] = my_max;
+    if (lane == 0) {
+        lds[wave] = my_max;
+        lds[8 + wave] = my_exp;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long m = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = lds[w] > m ? lds[w] : m;
+        unsigned long long m = 0, e = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            m = lds[w] > m ? lds[w] : m;
+            e = lds[8 + w] > e ? lds[8 + w] : e;
+        }
         if (m) atomicMax(&st->max_dv[iter], m);
+        if (e) atomicMax(&st->max_exp[iter], e);
     }
 }
 
@@ -472,24 +547,28 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
                                  const uint2* __restrict__ fcells, const V4<R>* __restrict__ xf,
                                  const V4<R>* __restrict__ fo, const V4<R>* __restrict__ vel,
                                  V4<R>* __restrict__ fp, V4<R>* __restrict__ xp, V4<R>* __restrict__ pos_out,
-                                 V4<R>* __restrict__ vel_out, unsigned long long* host_bits,
-                                 unsigned int* host_tag, unsigned long long* host_cand, unsigned int tag) {
+                                 V4<R>* __restrict__ vel_out, const R* __restrict__ p, HostMirror* host,
+                                 unsigned int tag, int warm_pass, V4<R>* __restrict__ xp_save) {
     using O = Ops<R, FAST>;
-    if (iteration_skipped<R>(st, iter, P)) return;
+    // warm_pass: the pressure force of the network's initial pressures, before the
+    // first prediction -- not an iteration of the loop.
+    if (!warm_pass && iteration_skipped<R>(st, iter, P)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        // Publish this iteration's max dv to host-visible memory (the pressure pass
-        // that produced it has completed): the host decides how far to enqueue
-        // without waiting on the stream.  Value first, then the step tag.
-        host_bits[iter] = st->max_dv[iter];
-        host_cand[0] = st->fluid_candidates;
-        host_cand[1] = st->boundary_candidates;
+    if (i == 0 && !warm_pass) {
+        // Publish this iteration's results to host-visible memory (the pressure pass
+        // that produced them has completed): the host decides how far to enqueue
+        // without waiting on the stream.  Values first, then the step tag.
+        host->max_dv[iter] = st->max_dv[iter];
+        host->max_exp[iter] = st->max_exp[iter];
+        host->candidates[0] = st->fluid_candidates;
+        host->candidates[1] = st->boundary_candidates;
+        host->warm_rejected = st->warm_rejected;
         __threadfence_system();
-        __hip_atomic_store(&host_tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&host->tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     if (i >= P.n) return;
-    const bool last = iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations, P.rho0,
-                                           P.threshold);
+    const bool last = !warm_pass && iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations,
+                                                         P.rho0, P.threshold);
     const V4<R> self = xf[i];
     const R sf = self.w;
     const int3 c = cell_of4<R>(self, P.cell);
@@ -519,13 +598,149 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
     V4<R> xo, vo;
     integrate<R, FAST>(P, self, v, o.x + f.x, o.y + f.y, o.z + f.z, xo, vo);
     if (!last) {
+        if (warm_pass) xp_save[i] = xp[i];  // keep the plain start's prediction
         xp[i] = xo;
     } else {
         pos_out[i] = xo;
+        vo.w = p[i];  // the converged pressure rides along to the next step
         vel_out[i] = vo;
         const bool bad = !(isfinite(xo.x) && isfinite(xo.y) && isfinite(xo.z) && isfinite(vo.x) &&
                            isfinite(vo.y) && isfinite(vo.z));
         if (bad) atomicOr(&st->nonfinite, 1u);
+    }
+}
+
+// First iteration of a warm-started step: the pressure update of pcisph.cl:212-263
+// for two starts at once -- the network's (xp, p) and the plain one (xp0, p = 0) --
+// sharing the candidate loop.  The step continues with whichever start has the
+// smaller residual max |rho* - rho0| (k_select_start), so a prediction that makes
+// the residual worse is dropped and the loop is exactly the plain one.
+template <typename R, bool FAST>
+__global__ void k_pressure_dual(SphParams<R> P, LoopState* __restrict__ st, const uint2* __restrict__ bcells,
+                                const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
+                                const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+                                const V4<R>* __restrict__ xp, R* __restrict__ dv, R* __restrict__ p,
+                                V4<R>* __restrict__ xf, R* __restrict__ bp, V4<R>* __restrict__ bxf,
+                                DualArrays<R> D) {
+    using O = Ops<R, FAST>;
+    __shared__ unsigned long long lds[4][8];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long m[4] = {0, 0, 0, 0};  // dv warm, exp warm, dv plain, exp plain
+    if (i < P.n) {
+        const V4<R> self = xr[i];
+        const V4<R> pw = xp[i], pz = D.xp0[i];
+        const int3 c = cell_of4<R>(self, P.cell);
+        R sw = 0, sz = 0;
+        if (P.nb)
+            for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
+                const V4<R> o = bpos[j];
+                R dx, dy, dz;
+                sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+            });
+        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+            const V4<R> ow = xp[j], oz = D.xp0[j];
+            R dx, dy, dz;
+            sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, ow.x, ow.y, ow.z, dx, dy, dz), P.h2);
+            sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, oz.x, oz.y, oz.z, dx, dy, dz), P.h2);
+        });
+        sw *= P.density_scale;
+        sz *= P.density_scale;
+        const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
+        m[0] = Bits<R>::of(vw);
+        m[1] = Bits<R>::of(fmax(R(0), P.rho0 - sw));
+        m[2] = Bits<R>::of(vz);
+        m[3] = Bits<R>::of(fmax(R(0), P.rho0 - sz));
+        dv[i] = vw;
+        D.dv0[i] = vz;
+        R pi = p[i];
+        if (vw != R(0)) {
+            pi += vw * P.delta;
+            p[i] = pi;
+        }
+        const R pz0 = vz != R(0) ? R(0) + vz * P.delta : R(0);
+        D.p0[i] = pz0;
+        V4<R> out = self;
+        out.w = O::div(pi, self.w * self.w);
+        xf[i] = out;
+        out.w = O::div(pz0, self.w * self.w);
+        D.xf0[i] = out;
+    } else if (i < P.n + P.nb) {
+        const uint32_t b = i - P.n;
+        const V4<R> self = bpos[b];
+        const int3 c = cell_of4<R>(self, P.cell);
+        R sw = 0, sz = 0;
+        sw += binit[b];
+        sz += binit[b];
+        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+            const V4<R> ow = xp[j], oz = D.xp0[j];
+            R dx, dy, dz;
+            sw += poly6<R>(dist2<R>(self.x, self.y, self.z, ow.x, ow.y, ow.z, dx, dy, dz), P.h2);
+            sz += poly6<R>(dist2<R>(self.x, self.y, self.z, oz.x, oz.y, oz.z, dx, dy, dz), P.h2);
+        });
+        sw *= P.density_scale;
+        sz *= P.density_scale;
+        const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
+        const R pbw = vw != R(0) ? bp[b] + vw * P.delta : bp[b];
+        const R pbz = vz != R(0) ? R(0) + vz * P.delta : R(0);
+        bp[b] = pbw;
+        D.bp0[b] = pbz;
+        V4<R> out = self;
+        out.w = O::div(pbw, P.rho0_sq);
+        bxf[b] = out;
+        out.w = O::div(pbz, P.rho0_sq);
+        D.bxf0[b] = out;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(m[q], off);
+            m[q] = o > m[q] ? o : m[q];
+        }
+        if (lane == 0) lds[q][wave] = m[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long v = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = lds[threadIdx.x][w] > v ? lds[threadIdx.x][w] : v;
+        // 0: dv warm, 1: expansion warm, 2: dv plain, 3: expansion plain
+        unsigned long long* dst[4] = {&st->dual_dv[0], &st->dual_exp[0], &st->dual_dv[1], &st->dual_exp[1]};
+        if (v) atomicMax(dst[threadIdx.x], v);
+    }
+}
+
+// Picks the start with the smaller residual and records its iteration-0 results
+// where the loop's termination logic reads them.
+template <typename R>
+__global__ void k_select_start(LoopState* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const R dw = Bits<R>::to(st->dual_dv[0]), ew = Bits<R>::to(st->dual_exp[0]);
+    const R dz = Bits<R>::to(st->dual_dv[1]), ez = Bits<R>::to(st->dual_exp[1]);
+    const R rw = dw > ew ? dw : ew, rz = dz > ez ? dz : ez;
+    const bool reject = !(rw < rz);
+    st->warm_rejected = reject ? 1u : 0u;
+    st->max_dv[0] = reject ? st->dual_dv[1] : st->dual_dv[0];
+    st->max_exp[0] = reject ? st->dual_exp[1] : st->dual_exp[0];
+}
+
+// After k_select_start rejected the network's start: make the plain start's
+// arrays the live ones (threads 0..n-1 fluid, n..n+nb-1 boundary).
+template <typename R>
+__global__ void k_adopt_plain_start(uint32_t n, uint32_t nb, const LoopState* __restrict__ st, DualArrays<R> D,
+                                    V4<R>* __restrict__ xp, V4<R>* __restrict__ xf, R* __restrict__ p,
+                                    R* __restrict__ dv, R* __restrict__ bp, V4<R>* __restrict__ bxf) {
+    if (!st->warm_rejected) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        xp[i] = D.xp0[i];
+        xf[i] = D.xf0[i];
+        p[i] = D.p0[i];
+        dv[i] = D.dv0[i];
+    } else if (i < n + nb) {
+        bp[i - n] = D.bp0[i - n];
+        bxf[i - n] = D.bxf0[i - n];
     }
 }
 

@@ -53,11 +53,15 @@ template <typename R, bool FAST>
 struct SphLaunch {
     static void density(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
     static void normal(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
-    static void force_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
+    static void force_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
+                           const FeatureArgs& F);
     static void pressure(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A, uint32_t iter);
     static void pressure_force(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
-                               uint32_t iter, unsigned long long* host_bits, unsigned int* host_tag,
-                               unsigned long long* host_cand, unsigned int tag);
+                               uint32_t iter, HostMirror* host, unsigned int tag, int warm_pass,
+                               V4<R>* xp_save);
+    // iteration 0 of a warm-started step (k_pressure_dual, k_select_start, k_adopt_plain_start)
+    static void pressure_dual(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
+                              const DualArrays<R>& D);
     static void boundary_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
 };
 

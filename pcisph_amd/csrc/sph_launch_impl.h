@@ -22,9 +22,10 @@ void SphLaunch<R, FAST>::normal(const Launch& L, const SphParams<R>& P, const De
 }
 
 template <typename R, bool FAST>
-void SphLaunch<R, FAST>::force_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A) {
+void SphLaunch<R, FAST>::force_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
+                                    const FeatureArgs& F) {
     hipLaunchKernelGGL((k_force_init<R, FAST>), dim3(grid_for(P.n, L.block)), dim3(L.block), 0, L.stream,
-                       P, A.cells, A.xr, A.vel, A.nrm, A.fo, A.p, A.fp, A.xp);
+                       P, A.cells, A.xr, A.vel, A.nrm, A.fo, A.p, A.fp, A.xp, F);
 }
 
 template <typename R, bool FAST>
@@ -38,11 +39,23 @@ void SphLaunch<R, FAST>::pressure(const Launch& L, const SphParams<R>& P, const 
 
 template <typename R, bool FAST>
 void SphLaunch<R, FAST>::pressure_force(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
-                                        uint32_t iter, unsigned long long* host_bits, unsigned int* host_tag,
-                                        unsigned long long* host_cand, unsigned int tag) {
+                                        uint32_t iter, HostMirror* host, unsigned int tag, int warm_pass,
+                                        V4<R>* xp_save) {
     hipLaunchKernelGGL((k_pressure_force<R, FAST>), dim3(grid_for(P.n, L.block)), dim3(L.block), 0,
                        L.stream, P, iter, A.state, A.bcells, A.bxf, A.cells, A.xf, A.fo, A.vel, A.fp, A.xp,
-                       A.pos_in, A.vel_in, host_bits, host_tag, host_cand, tag);
+                       A.pos_in, A.vel_in, A.p, host, tag, warm_pass, xp_save);
+}
+
+template <typename R, bool FAST>
+void SphLaunch<R, FAST>::pressure_dual(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
+                                       const DualArrays<R>& D) {
+    const uint32_t total = P.n + P.nb;
+    hipLaunchKernelGGL((k_pressure_dual<R, FAST>), dim3(grid_for(total, L.block)), dim3(L.block), 0, L.stream,
+                       P, A.state, A.bcells, A.bpos, A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp, A.bxf,
+                       D);
+    hipLaunchKernelGGL((k_select_start<R>), dim3(1), dim3(64), 0, L.stream, A.state);
+    hipLaunchKernelGGL((k_adopt_plain_start<R>), dim3(grid_for(total, 256)), dim3(256), 0, L.stream, P.n, P.nb,
+                       A.state, D, A.xp, A.xf, A.p, A.dv, A.bp, A.bxf);
 }
 
 template <typename R, bool FAST>

@@ -106,3 +106,41 @@ def test_ieee_matches_golden(gpu, name, idx):
         if k in ("iterations", "max_density_variation", "params"):
             continue
         np.testing.assert_array_equal(g.download(Field[k]), ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("b3", [0.0, 100.0])
+def test_warm_start_fallback_is_the_plain_loop(gpu, oracle_lib, b3):
+    """A network whose start is no better than the plain one (zero output) or much
+    worse (a huge uniform pressure) is rejected at the first iteration, and the step
+    is then exactly the plain step (IEEE FP32, bit for bit against the oracle)."""
+    from oracle import mlp_oracle as M
+    scene = small_scenes()[0]
+    print(f"case warm fallback b3={b3}", flush=True)
+    p = np.zeros(M.N_PARAMS, np.float32)
+    p[-1] = b3
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    g.load_scene(scene, max_iterations=20)
+    g.mlp_set_params(p)
+    g.set_warm_start(True)
+    o = oracle_lib.OracleFluid()
+    o.load_scene(scene, max_iterations=20)
+    for _ in range(3):
+        a, b = g.step(), o.step()
+        assert a["warm_start"] == 3, a
+        assert a["iterations"] == b["iterations"]
+    for f in (Field.FLUID_POSITIONS, Field.FLUID_VELOCITIES, Field.FLUID_PRESSURES, Field.FLUID_DENSITIES):
+        np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
+
+
+def test_sample_collection(gpu):
+    scene = small_scenes()[0]
+    g = gpu.Fluid()
+    g.load_scene(scene, max_iterations=20)
+    g.samples_reserve(10000)
+    g.samples_collect(4)
+    for _ in range(3):
+        g.step()
+    x, t = g.samples_download()
+    assert x.shape == (3 * 128, 8) and np.all(np.isfinite(x)) and np.all(t >= 0)
+    # feature 1 (fluid neighbours / 32) and 6 (boundary neighbours / 32) are counts
+    assert np.allclose(x[:, 1] * 32, np.round(x[:, 1] * 32)) and np.allclose(x[:, 6] * 32, np.round(x[:, 6] * 32))
