@@ -243,6 +243,13 @@ int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float lea
                      float* last_loss);
 int pcisph_set_warm_start(pcisph_solver* s, int enable);
 
+/* Autotuner: times `trial_steps` steps of the current scene for each threads-per-block
+ * choice (64..512) and radix digit width (5..8 bits) from the same saved starting state,
+ * keeps the fastest in the solver options (and returns them in `chosen` if non-NULL),
+ * and restores the state.  No counterpart in the reference (clogs has its own offline
+ * tuner, exec_clogs_tune.bat). */
+int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen);
+
 /* Derived parameters for a parameter set without a device (host only). */
 int pcisph_compute_params(float particle_radius, float rest_density, uint32_t boundary_count,
                           uint32_t fluid_count, float viscosity, float surface_tension,

@@ -36,6 +36,7 @@ EXPORTS = (
     "pcisph_mlp_set_params", "pcisph_mlp_get_params", "pcisph_mlp_forward", "pcisph_mlp_gradients",
     "pcisph_samples_reserve", "pcisph_samples_collect", "pcisph_samples_count",
     "pcisph_samples_download", "pcisph_samples_upload", "pcisph_mlp_train", "pcisph_set_warm_start",
+    "pcisph_autotune",
 )
 
 MLP_INPUTS = 8
@@ -192,6 +193,12 @@ class Fluid(SolverBase):
         self._call("mlp_train", C.c_uint32(steps), C.c_uint32(batch), C.c_float(lr), C.c_uint64(seed),
                    C.byref(loss))
         return loss.value
+
+    def autotune(self, trial_steps: int = 3) -> dict:
+        """Pick block size and radix digit width by timing trial steps (state unchanged)."""
+        o = SolverOptions()
+        self._call("autotune", C.c_uint32(trial_steps), C.byref(o))
+        return {"block_size": o.block_size, "sort_digit_bits": o.sort_digit_bits}
 
     def set_warm_start(self, on: bool = True):
         self._call("set_warm_start", C.c_int(1 if on else 0))

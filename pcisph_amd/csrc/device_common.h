@@ -61,7 +61,7 @@ __device__ __forceinline__ int32_t sat_floor(R v) {
     return (int32_t)f;
 }
 
-__device__ __forceinline__ float div_rn(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ float div_rn(float a, float b) { return a / b; }  // correctly rounded
 __device__ __forceinline__ double div_rn(double a, double b) { return a / b; }
 
 // The cell of a position: IEEE division in every mode, so keys are bit-exact.

@@ -248,6 +248,10 @@ class EngineBase {
     virtual size_t field_size(int field, const Simulation_Params& p) const = 0;
     virtual void profile_enable(bool on) = 0;
     virtual void profile_read(pcisph_stage_times* out) = 0;
+    // device-side copy of the state a step starts from (positions, velocities with the
+    // previous pressures, boundary positions), for trial runs of the autotuner
+    virtual void snapshot() = 0;
+    virtual void restore() = 0;
 };
 
 size_t field_size_for(int field, uint32_t n, uint32_t nb, uint32_t B) {
@@ -307,6 +311,7 @@ class Engine final : public EngineBase {
         dfree(A_.state);
         dfree(staging_);
         dfree(feat_);
+        drop_snapshot();
         dfree(D_.xp0);
         dfree(D_.xf0);
         dfree(D_.p0);
@@ -392,6 +397,30 @@ class Engine final : public EngineBase {
 
     void zero_velocities() override {
         if (A_.n) HIP_CHECK(hipMemsetAsync(A_.vel_in, 0, sizeof(V4<R>) * A_.n, stream_));
+    }
+
+    void snapshot() override {
+        dalloc(snap_pos_, std::max(A_.n, 1u), stream_);
+        dalloc(snap_vel_, std::max(A_.n, 1u), stream_);
+        dalloc(snap_b_, std::max(A_.nb, 1u), stream_);
+        HIP_CHECK(hipMemcpyAsync(snap_pos_, A_.pos_in, sizeof(V4<R>) * A_.n, hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(snap_vel_, A_.vel_in, sizeof(V4<R>) * A_.n, hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(snap_b_, A_.bpos, sizeof(V4<R>) * A_.nb, hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void restore() override {
+        if (!snap_pos_) throw Error(PCISPH_ERR_INTERNAL, "no snapshot");
+        HIP_CHECK(hipMemcpyAsync(A_.pos_in, snap_pos_, sizeof(V4<R>) * A_.n, hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(A_.vel_in, snap_vel_, sizeof(V4<R>) * A_.n, hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(A_.bpos, snap_b_, sizeof(V4<R>) * A_.nb, hipMemcpyDeviceToDevice, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void drop_snapshot() {
+        dfree(snap_pos_);
+        dfree(snap_vel_);
+        dfree(snap_b_);
     }
 
     size_t field_size(int field, const Simulation_Params& p) const override {
@@ -648,6 +677,9 @@ class Engine final : public EngineBase {
     size_t staging_words_ = 0;
     float* feat_ = nullptr;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0;
+    V4<R>* snap_pos_ = nullptr;
+    V4<R>* snap_vel_ = nullptr;
+    V4<R>* snap_b_ = nullptr;
     volatile HostMirror* host_ = nullptr;
     HostMirror* dev_host_ = nullptr;
     unsigned int tag_ = 0;
@@ -1118,6 +1150,54 @@ int pcisph_set_warm_start(pcisph_solver* s, int enable) {
     return guarded(s, [&] {
         if (enable) s->ensure_mlp();
         s->warm_start = enable != 0;
+    });
+}
+
+// ------------------------------------------------------------------ autotuner
+// Times `trial_steps` steps of the current scene for each block size of the particle
+// kernels and each radix digit width, from the same starting state each time (a
+// device snapshot restored after every trial), then keeps the fastest pair in the
+// solver options.  The state after the call is the state before it.
+int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen) {
+    return guarded(s, [&] {
+        if (!s->params.fluid_count) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "no particles to tune on");
+        if (!trial_steps) trial_steps = 3;
+        apply_param_changes(s);
+        // the boundary sort a pending update would do is part of every trial's first step
+        const bool boundary_pending = s->boundary_updated;
+        s->eng->snapshot();
+        const pcisph_solver_options base = s->opt;
+        const uint32_t collect = s->collect_stride;
+        s->collect_stride = 0;
+        double best = 1e300;
+        pcisph_solver_options best_opt = base;
+        auto trial = [&](uint32_t block, uint32_t digit) {
+            s->opt = base;
+            s->opt.block_size = block;
+            s->opt.sort_digit_bits = digit;
+            s->eng->restore();
+            s->boundary_updated = boundary_pending;
+            step_impl(s, nullptr);  // warm-up: caches, code objects
+            s->eng->restore();
+            s->boundary_updated = boundary_pending;
+            HIP_CHECK(hipStreamSynchronize(s->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            for (uint32_t i = 0; i < trial_steps; ++i) step_impl(s, nullptr);
+            HIP_CHECK(hipStreamSynchronize(s->stream));
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (dt < best) {
+                best = dt;
+                best_opt = s->opt;
+            }
+        };
+        for (uint32_t block : {64u, 128u, 256u, 512u}) trial(block, 8u);
+        const uint32_t block = best_opt.block_size;
+        for (uint32_t digit : {5u, 6u, 7u}) trial(block, digit);
+        s->opt = best_opt;
+        s->collect_stride = collect;
+        s->eng->restore();
+        s->boundary_updated = boundary_pending;
+        if (chosen) *chosen = best_opt;
     });
 }
 

@@ -164,8 +164,11 @@ template <typename R, bool FAST>
 struct Ops;
 template <>
 struct Ops<float, false> {
-    __device__ static float div(float a, float b) { return __fdiv_rn(a, b); }
-    __device__ static float sqrt(float a) { return __fsqrt_rn(a); }
+    // `/` and __builtin_sqrtf are correctly rounded in HIP's default mode (the
+    // v_sqrt_f32 result is corrected by two FMAs); __fsqrt_rn is NOT: without
+    // OCML_BASIC_ROUNDED_OPERATIONS it maps to the native (1 ulp) square root.
+    __device__ static float div(float a, float b) { return a / b; }
+    __device__ static float sqrt(float a) { return __builtin_sqrtf(a); }
 };
 template <>
 struct Ops<float, true> {
