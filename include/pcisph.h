@@ -243,6 +243,11 @@ int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float lea
                      float* last_loss);
 int pcisph_set_warm_start(pcisph_solver* s, int enable);
 
+/* Rigid motion of the walls (moving tanks): adds (dx, dy, dz) to every boundary
+ * position on the device and re-sorts the boundary at the next step.  No counterpart
+ * in the reference, whose boundary never moves. */
+int pcisph_translate_boundary(pcisph_solver* s, float dx, float dy, float dz);
+
 /* Autotuner: times `trial_steps` steps of the current scene for each threads-per-block
  * choice (64..512) and radix digit width (5..8 bits) from the same saved starting state,
  * keeps the fastest in the solver options (and returns them in `chosen` if non-NULL),

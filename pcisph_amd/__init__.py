@@ -36,7 +36,7 @@ EXPORTS = (
     "pcisph_mlp_set_params", "pcisph_mlp_get_params", "pcisph_mlp_forward", "pcisph_mlp_gradients",
     "pcisph_samples_reserve", "pcisph_samples_collect", "pcisph_samples_count",
     "pcisph_samples_download", "pcisph_samples_upload", "pcisph_mlp_train", "pcisph_set_warm_start",
-    "pcisph_autotune",
+    "pcisph_autotune", "pcisph_translate_boundary",
 )
 
 MLP_INPUTS = 8
@@ -72,6 +72,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.pcisph_mlp_train.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.c_void_p]
     L.pcisph_samples_collect.argtypes = [C.c_void_p, C.c_uint32]
     L.pcisph_set_warm_start.argtypes = [C.c_void_p, C.c_int]
+    L.pcisph_translate_boundary.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
     _lib = L
     return L
 
@@ -199,6 +200,10 @@ class Fluid(SolverBase):
         o = SolverOptions()
         self._call("autotune", C.c_uint32(trial_steps), C.byref(o))
         return {"block_size": o.block_size, "sort_digit_bits": o.sort_digit_bits}
+
+    def translate_boundary(self, dx: float, dy: float, dz: float):
+        """Move the walls rigidly by (dx, dy, dz); the boundary is re-sorted next step."""
+        self._call("translate_boundary", dx, dy, dz)
 
     def set_warm_start(self, on: bool = True):
         self._call("set_warm_start", C.c_int(1 if on else 0))

@@ -250,6 +250,7 @@ class EngineBase {
     virtual void profile_read(pcisph_stage_times* out) = 0;
     // device-side copy of the state a step starts from (positions, velocities with the
     // previous pressures, boundary positions), for trial runs of the autotuner
+    virtual void translate_boundary(float dx, float dy, float dz) = 0;
     virtual void snapshot() = 0;
     virtual void restore() = 0;
 };
@@ -397,6 +398,11 @@ class Engine final : public EngineBase {
 
     void zero_velocities() override {
         if (A_.n) HIP_CHECK(hipMemsetAsync(A_.vel_in, 0, sizeof(V4<R>) * A_.n, stream_));
+    }
+
+    void translate_boundary(float dx, float dy, float dz) override {
+        DataLaunch<R>::translate(Launch{stream_, 256}, A_.bpos, A_.nb, dx, dy, dz);
+        HIP_CHECK(hipGetLastError());
     }
 
     void snapshot() override {
@@ -1143,6 +1149,13 @@ int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float lr,
         HIP_CHECK(hipMemcpyAsync(&loss, s->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, s->stream));
         HIP_CHECK(hipStreamSynchronize(s->stream));
         if (last_loss) *last_loss = loss / (float)batch;
+    });
+}
+
+int pcisph_translate_boundary(pcisph_solver* s, float dx, float dy, float dz) {
+    return guarded(s, [&] {
+        s->eng->translate_boundary(dx, dy, dz);
+        s->boundary_updated = true;
     });
 }
 

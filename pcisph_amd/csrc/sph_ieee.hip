@@ -45,7 +45,24 @@ __global__ void k_pack_scalar(const R* __restrict__ src, O* __restrict__ dst, ui
     dst[i] = (O)src[(size_t)i * stride + offset];
 }
 
+template <typename R>
+__global__ void k_translate(V4<R>* __restrict__ p, uint32_t n, R dx, R dy, R dz) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V4<R> v = p[i];
+    v.x = v.x + dx;
+    v.y = v.y + dy;
+    v.z = v.z + dz;
+    p[i] = v;
+}
+
 }  // namespace
+
+template <typename R>
+void DataLaunch<R>::translate(const Launch& L, V4<R>* p, uint32_t n, float dx, float dy, float dz) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_translate<R>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, p, n, (R)dx, (R)dy, (R)dz);
+}
 
 template <typename R>
 void DataLaunch<R>::hash(const Launch& L, const V4<R>* pos, uint32_t n, R cell, const HashGrid& g,

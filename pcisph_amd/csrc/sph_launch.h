@@ -83,6 +83,8 @@ struct DataLaunch {
                             uint32_t offset);
     static void pack_scalar_f64(const Launch& L, const R* src, double* dst, uint32_t n, uint32_t stride,
                                 uint32_t offset);
+    // p[i].xyz += (dx, dy, dz) in precision R
+    static void translate(const Launch& L, V4<R>* p, uint32_t n, float dx, float dy, float dz);
 };
 
 void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);

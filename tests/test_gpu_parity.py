@@ -14,9 +14,9 @@ from tests.helpers import (FLOAT_FIELDS, INT_FIELDS, max_rel_err, run, small_sce
 
 pytestmark = pytest.mark.gpu
 
-# fast FP32 vs oracle, after one step: relative error per field, with an absolute floor
-# of 1e-3 of the field's scale (fields near zero, e.g. normals inside the fluid, are
-# compared against that floor).
+# fast FP32 vs the FP32 oracle after one step: max abs difference of a field over the
+# field's largest magnitude (velocities after one step are small differences of large
+# forces, so a per-element relative test would measure cancellation, not the kernels)
 FAST_RTOL = 2e-4
 
 
@@ -61,22 +61,33 @@ def test_fp64_bit_exact(gpu, oracle_lib):
 
 @pytest.mark.parametrize("scene_idx", [0, 1])
 def test_fast_fp32_within_tolerance(gpu, oracle_lib, scene_idx):
+    """Fast FP32 after one step, per field: max |fast - oracle| / max |oracle| below
+    FAST_RTOL.  Also reports both FP32 runs' error against the FP64 oracle."""
     scene = small_scenes()[scene_idx]
     print(f"case fast_fp32 {scene.name}", flush=True)
     o = oracle_lib.OracleFluid()
+    o64 = oracle_lib.OracleFluid(fp64=True)
     g = gpu.Fluid()
     so = run(o, scene, 1, max_iterations=20)
+    run(o64, scene, 1, max_iterations=20)
     sg = run(g, scene, 1, max_iterations=20)
     assert so[0]["iterations"] == sg[0]["iterations"]
     ro, rg = snapshot(o), snapshot(g)
     for f in INT_FIELDS:
         np.testing.assert_array_equal(rg[Field(f).name], ro[Field(f).name], err_msg=Field(f).name)
+    errs = {}
     for f in FLOAT_FIELDS:
         name = Field(f).name
-        ref = ro[name]
-        floor = 1e-3 * max(float(np.max(np.abs(ref))), 1e-30)
-        err = max_rel_err(rg[name], ref, floor)
-        assert err < FAST_RTOL, f"{name}: max rel err {err:.3e}"
+        ref = ro[name].astype(np.float64)
+        r64 = o64.download_f64(f)
+        scale = max(float(np.max(np.abs(r64))), 1e-30)
+        errs[name] = (float(np.max(np.abs(rg[name] - ref))) / scale,
+                      float(np.max(np.abs(rg[name] - r64))) / scale,
+                      float(np.max(np.abs(ref - r64))) / scale)
+        print(f"  {name:28s} fast-vs-ieee {errs[name][0]:.2e}  fast-vs-fp64 {errs[name][1]:.2e}  "
+              f"ieee-vs-fp64 {errs[name][2]:.2e}", flush=True)
+    bad = {k: v for k, v in errs.items() if not v[0] < FAST_RTOL}
+    assert not bad, bad
 
 
 def test_iteration_bounds(gpu, oracle_lib):
