@@ -89,7 +89,14 @@ typedef struct pcisph_solver_options {
     uint32_t block_size;            /* threads per block of the per-particle kernels;
                                        0 = autotuned/default */
     uint32_t sort_digit_bits;       /* radix digit width, 4..8; 0 = autotuned/default */
-    uint32_t reserved[2];
+    float neighbor_skin;            /* neighbour lists: each step the density pass records, in
+                                       the reference's summation order, every candidate within
+                                       (1 + skin) h; later passes walk those lists instead of
+                                       the 27 buckets.  Results are identical (the skipped
+                                       candidates contribute exact zeros); an iteration whose
+                                       predicted positions moved more than 0.45 skin h falls
+                                       back to the bucket walk.  0 = off; default 0.2 */
+    uint32_t reserved;
 } pcisph_solver_options;
 
 /* Per-step report (the reference only has a compiled-out timer, Fluid.cpp:319-326). */

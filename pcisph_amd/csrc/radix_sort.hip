@@ -4,10 +4,11 @@
 // sort (libs/clogs/include/clogs/radixsort.h:182-184).  Written from that contract
 // and the published reduce-then-scan radix sort (Merrill & Grimshaw 2011).
 //
-// Per digit pass, three launches:
+// Per digit pass:
 //   upsweep   : each tile of 2048 keys builds its digit histogram in LDS and writes
 //               it digit-major, hist[d * tiles + t]
-//   scan      : one workgroup turns hist into exclusive global offsets
+//   scan      : hist becomes exclusive global offsets (three launches: chunk scans,
+//               scan of the chunk totals, add-back)
 //   downsweep : each tile re-reads its keys, ranks them stably with wavefront
 //               ballots (the lanes sharing a digit are found with one ballot per
 //               digit bit; a lane's rank is the popcount of those below it), adds
@@ -43,29 +44,69 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
     for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d * tiles + blockIdx.x] = s_hist[d];
 }
 
-// Exclusive scan of `count` words by one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ data, uint32_t count) {
-    __shared__ uint32_t s_part[1024];
-    const uint32_t per = (count + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per;
-    const uint32_t hi = min(count, lo + per);
-    uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; ++i) sum += data[i];
-    s_part[threadIdx.x] = sum;
+// Exclusive scan of `count` words in three launches: chunks of kScanChunk words
+// scanned by one workgroup each (k_scan_chunks, chunk totals to `partials`), the
+// totals scanned by one workgroup (k_scan_totals), the totals added back
+// (k_scan_add).
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr int kScanChunk = kScanThreads * kScanPer;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
     __syncthreads();
-    // Hillis-Steele inclusive scan of the 1024 partial sums
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        if (w < wave) before += s_tmp[w];
+        total += s_tmp[w];
     }
-    uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
+    __syncthreads();
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_chunks(uint32_t* __restrict__ data, uint32_t count,
+                                                              uint32_t* __restrict__ partials) {
+    __shared__ uint32_t s_tmp[kScanThreads / 64];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+    uint32_t v[kScanPer], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = base + k < count ? data[base + k] : 0u;
+        sum += v[k];
     }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, s_tmp, total);
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        if (base + k < count) data[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_totals(uint32_t* __restrict__ partials, uint32_t chunks) {
+    __shared__ uint32_t s_tmp[16];
+    const uint32_t v = threadIdx.x < chunks ? partials[threadIdx.x] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, s_tmp, total);
+    if (threadIdx.x < chunks) partials[threadIdx.x] = ex;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* __restrict__ data, uint32_t count,
+                                                           const uint32_t* __restrict__ partials) {
+    const uint32_t add = partials[blockIdx.x];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k)
+        if (base + k < count) data[base + k] += add;
 }
 
 __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restrict__ kin,
@@ -154,7 +195,8 @@ void fill_u64(const Launch& L, unsigned long long* p, size_t n, unsigned long lo
 
 size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits) {
     const size_t tiles = (n + kTile - 1) / kTile;
-    return (size_t(1) << digit_bits) * (tiles ? tiles : 1);
+    const size_t words = (size_t(1) << digit_bits) * (tiles ? tiles : 1);
+    return words + 1024;  // + the chunk totals of the scan
 }
 
 int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
@@ -167,7 +209,12 @@ int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n
         const uint32_t words = (1u << bits) * tiles;
         hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], n, shift, bits,
                            scratch, tiles);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, L.stream, scratch, words);
+        const uint32_t chunks = (words + kScanChunk - 1) / kScanChunk;
+        if (chunks > 1024) return -1;  // more than 2^24 keys at 8-bit digits
+        uint32_t* partials = scratch + (radix_sort_scratch_words(n, digit_bits) - 1024);
+        hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(kScanThreads), 0, L.stream, scratch, words, partials);
+        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, L.stream, partials, chunks);
+        hipLaunchKernelGGL(k_scan_add, dim3(chunks), dim3(kScanThreads), 0, L.stream, scratch, words, partials);
         hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], vals[cur],
                            keys[cur ^ 1], vals[cur ^ 1], n, shift, bits, scratch, tiles);
         cur ^= 1;

@@ -313,6 +313,12 @@ class Engine final : public EngineBase {
         dfree(staging_);
         dfree(feat_);
         drop_snapshot();
+        dfree(A_.NL.f);
+        dfree(A_.NL.b);
+        dfree(A_.NL.bf);
+        dfree(A_.NL.fcount);
+        dfree(A_.NL.bcount);
+        dfree(A_.NL.bfcount);
         dfree(D_.xp0);
         dfree(D_.xf0);
         dfree(D_.p0);
@@ -553,12 +559,14 @@ class Engine final : public EngineBase {
         const bool warm = w.params != nullptr;
         const bool features = warm || w.collect_stride;
         if (features) ensure_warm_buffers(warm);
+        setup_lists(o.neighbor_skin, P);
         fill_u64(L, (unsigned long long*)A_.state, sizeof(LoopState) / 8, 0ull);
         if (sort_boundary && A_.nb) {
             prof_.timed(PCISPH_STAGE_BOUNDARY_SORT, stream_, -1, [&] {
                 fill_u32(L, (uint32_t*)A_.bcells, 2 * (size_t)A_.B, 0u);
                 DataLaunch<R>::hash(L, A_.bpos, A_.nb, P.cell, P.grid, A_.bkeys[0], A_.bvals[0]);
                 boundary_sorted_ = radix_sort(L, A_.bkeys, A_.bvals, A_.nb, key_bits, digit, sort_scratch_);
+                if (boundary_sorted_ < 0) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "too many boundary particles");
                 DataLaunch<R>::reorder(L, A_.nb, A_.bkeys[boundary_sorted_], A_.bvals[boundary_sorted_],
                                        A_.bpos, nullptr, A_.bpos_tmp, nullptr, A_.bcells);
                 std::swap(A_.bpos, A_.bpos_tmp);
@@ -572,6 +580,7 @@ class Engine final : public EngineBase {
         prof_.timed(PCISPH_STAGE_SORT, stream_, -1, [&] {
             fluid_sorted_ = radix_sort(L, A_.keys, A_.vals, A_.n, key_bits, digit, sort_scratch_);
         });
+        if (fluid_sorted_ < 0) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "too many particles for one sort");
         prof_.timed(PCISPH_STAGE_REORDER, stream_, -1, [&] {
             DataLaunch<R>::reorder(L, A_.n, A_.keys[fluid_sorted_], A_.vals[fluid_sorted_], A_.pos_in,
                                    A_.vel_in, A_.pos, A_.vel, A_.cells);
@@ -641,6 +650,33 @@ class Engine final : public EngineBase {
         return r;
     }
 
+    // Neighbour-list storage (capacities: fluid 128, boundary 96 per fluid particle,
+    // fluid 96 per boundary particle) and this step's radius and validity limit.
+    void setup_lists(float skin, const SphParams<R>& P) {
+        NeighborLists& NL = A_.NL;
+        if (!(skin > 0.f)) {
+            NL.r2 = 0.f;
+            return;
+        }
+        constexpr uint32_t kCapF = 128, kCapB = 96, kCapBF = 96;
+        if (list_n_ != A_.n || list_nb_ != A_.nb || !NL.f) {
+            dalloc(NL.f, (size_t)kCapF * std::max(A_.n, 1u), stream_);
+            dalloc(NL.b, (size_t)kCapB * std::max(A_.n, 1u), stream_);
+            dalloc(NL.bf, (size_t)kCapBF * std::max(A_.nb, 1u), stream_);
+            dalloc(NL.fcount, std::max(A_.n, 1u), stream_);
+            dalloc(NL.bcount, std::max(A_.n, 1u), stream_);
+            dalloc(NL.bfcount, std::max(A_.nb, 1u), stream_);
+            NL.cap_f = kCapF;
+            NL.cap_b = kCapB;
+            NL.cap_bf = kCapBF;
+            list_n_ = A_.n;
+            list_nb_ = A_.nb;
+        }
+        const float radius = (1.f + skin) * (float)P.h;
+        NL.r2 = radius * radius;
+        NL.disp_limit = 0.45f * skin * (float)P.h;
+    }
+
     void ensure_warm_buffers(bool dual) {
         const uint32_t n = A_.n, nb = A_.nb;
         if (feat_n_ != n) {
@@ -682,7 +718,7 @@ class Engine final : public EngineBase {
     float* staging_ = nullptr;
     size_t staging_words_ = 0;
     float* feat_ = nullptr;
-    uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0;
+    uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
     V4<R>* snap_pos_ = nullptr;
     V4<R>* snap_vel_ = nullptr;
     V4<R>* snap_b_ = nullptr;
@@ -728,7 +764,7 @@ struct pcisph_solver {
     bool params_changed = true;
     bool boundary_updated = true;
     float threshold = 0.01f;
-    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, {0, 0}};
+    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 0};
     int precision = kPrecisionFast;
     std::unique_ptr<EngineBase> eng;
     std::string err;
@@ -878,6 +914,8 @@ int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64, at most 512");
         if (opt->sort_digit_bits && (opt->sort_digit_bits < 4 || opt->sort_digit_bits > 8))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "sort_digit_bits must be in 4..8");
+        if (!(opt->neighbor_skin >= 0.f && opt->neighbor_skin <= 2.f))
+            throw Error(PCISPH_ERR_INVALID_ARGUMENT, "neighbor_skin must be in [0, 2]");
         s->opt = *opt;
     });
 }

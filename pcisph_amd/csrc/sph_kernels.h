@@ -52,6 +52,7 @@ struct SphParams {
 struct LoopState {
     unsigned long long max_dv[kMaxIterationsCap];
     unsigned long long max_exp[kMaxIterationsCap];  // max(0, rho0 - rho*): the signed residual's other side
+    unsigned int max_disp[kMaxIterationsCap];        // float bits: max |xp - x| entering iteration k
     unsigned long long fluid_candidates;
     unsigned long long boundary_candidates;
     // iteration 0 of a warm-started step evaluates both starts: [0] warm, [1] plain
@@ -225,6 +226,43 @@ __device__ __forceinline__ void for_each_candidate(const uint2* __restrict__ cel
     });
 }
 
+// Per-step neighbour lists, slot-major (entry k of particle i at [k * stride + i], so
+// the lanes of a wavefront read consecutive words): for every fluid particle its
+// fluid and boundary candidates within the list radius, for every boundary particle
+// its fluid candidates, each in the reference's bucket order.  A particle with more
+// neighbours than the capacity gets kListOverflow and walks its buckets.
+constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
+
+struct NeighborLists {
+    uint32_t* f = nullptr;        // fluid neighbours of fluid particles    [cap_f][n]
+    uint32_t* b = nullptr;        // boundary neighbours of fluid particles [cap_b][n]
+    uint32_t* bf = nullptr;       // fluid neighbours of boundary particles [cap_bf][nb]
+    uint32_t* fcount = nullptr;   // [n]
+    uint32_t* bcount = nullptr;   // [n]
+    uint32_t* bfcount = nullptr;  // [nb]
+    uint32_t cap_f = 0, cap_b = 0, cap_bf = 0;
+    float r2 = 0.f;          // list radius squared; 0: no lists
+    float disp_limit = 0.f;  // largest max |xp - x| with which a pressure pass may use them
+};
+
+struct ListWriter {
+    uint32_t* base;
+    uint32_t stride, i, cap, len;
+    __device__ __forceinline__ void push(uint32_t j) {
+        if (len < cap) base[(size_t)len * stride + i] = j;
+        ++len;
+    }
+    __device__ __forceinline__ uint32_t result() const { return len <= cap ? len : kListOverflow; }
+};
+
+template <typename F>
+__device__ __forceinline__ void for_each_listed(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
+                                                uint32_t i, F&& body) {
+    const uint32_t* q = list + i;
+#pragma unroll 2
+    for (uint32_t k = 0; k < len; ++k) body(q[(size_t)k * stride]);
+}
+
 template <typename T>
 __device__ __forceinline__ T block_reduce_sum(T v, T* lds) {
 #pragma unroll
@@ -268,22 +306,55 @@ __global__ void k_reorder(uint32_t n, const uint32_t* __restrict__ keys, const u
     if (d == n - 1 || k != keys[d + 1]) cells[k].y = d + 1;
 }
 
-// update_density (pcisph.cl:65-92), plus per-particle candidate / neighbour counts.
-// Boundary particle pressures are reset here too (boundary_pressure_initialization,
-// pcisph.cl:114-119): threads n .. n+nb-1 handle the boundary.
+// Wave-wide maximum of a non-negative float, folded into *dst (uint bits order =
+// float order for x >= 0).  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_max_to(unsigned int* dst, float v) {
+    unsigned int b = __float_as_uint(v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int o = __shfl_xor(b, off);
+        b = o > b ? o : b;
+    }
+    if ((threadIdx.x & 63) == 0 && b) atomicMax(dst, b);
+}
+
+template <typename R>
+__device__ __forceinline__ float displacement(const V4<R>& a, const V4<R>& b) {
+    const float dx = (float)(a.x - b.x), dy = (float)(a.y - b.y), dz = (float)(a.z - b.z);
+    return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// Walks particle i's list if it has one, else its 27 buckets (same order, same
+// non-zero terms).
+template <typename R, typename F>
+__device__ __forceinline__ void for_neighbours(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
+                                               uint32_t i, const uint2* __restrict__ cells, int3 c,
+                                               const HashGrid& g, F&& body) {
+    if (len != kListOverflow) for_each_listed(list, len, stride, i, body);
+    else for_each_candidate<R>(cells, c, g, body);
+}
+
+// update_density (pcisph.cl:65-92), plus per-particle candidate / neighbour counts,
+// plus this step's neighbour lists (every candidate within the list radius, in the
+// reference's bucket order).  Boundary particle pressures are reset here too
+// (boundary_pressure_initialization, pcisph.cl:114-119): threads n .. n+nb-1 handle
+// the boundary and record their fluid neighbours.
 template <typename R, bool FAST>
 __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
                           const uint2* __restrict__ fcells, const V4<R>* __restrict__ pos,
                           V4<R>* __restrict__ xr, uint4* __restrict__ counts, R* __restrict__ bp,
-                          V4<R>* __restrict__ bxf, LoopState* __restrict__ st) {
+                          V4<R>* __restrict__ bxf, LoopState* __restrict__ st, NeighborLists NL) {
     __shared__ unsigned long long lds[16];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool lists = NL.r2 > 0.f;
+    const R lr2 = (R)NL.r2;
     unsigned long long cf_sum = 0, cb_sum = 0;
     if (i < P.n) {
         const V4<R> self = pos[i];
         const int3 c = cell_of4<R>(self, P.cell);
         R rho = 0;
         uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
+        ListWriter wb{NL.b, P.n, i, NL.cap_b, 0}, wf{NL.f, P.n, i, NL.cap_f, 0};
         if (P.nb)
             for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
                 const V4<R> o = bpos[j];
@@ -292,6 +363,7 @@ __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, cons
                 rho += poly6<R>(r2, P.h2);
                 ++cb;
                 nb += !(r2 > P.h2);
+                if (lists && r2 <= lr2) wb.push(j);
             });
         for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
             const V4<R> o = pos[j];
@@ -300,12 +372,17 @@ __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, cons
             rho += poly6<R>(r2, P.h2);
             ++cf;
             nf += !(r2 > P.h2);
+            if (lists && r2 <= lr2) wf.push(j);
         });
         rho *= P.density_scale;
         V4<R> out = self;
         out.w = rho;
         xr[i] = out;
         if (counts) counts[i] = make_uint4(cf, cb, nf, nb);
+        if (lists) {
+            NL.bcount[i] = wb.result();
+            NL.fcount[i] = wf.result();
+        }
         cf_sum = cf;
         cb_sum = cb;
     } else if (i < P.n + P.nb) {
@@ -314,6 +391,16 @@ __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, cons
         V4<R> o = bpos[b];
         o.w = 0;
         bxf[b] = o;
+        if (lists) {
+            const int3 c = cell_of4<R>(o, P.cell);
+            ListWriter w{NL.bf, P.nb, b, NL.cap_bf, 0};
+            for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+                const V4<R> q = pos[j];
+                R dx, dy, dz;
+                if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
+            });
+            NL.bfcount[b] = w.result();
+        }
     }
     const unsigned long long a = block_reduce_sum(cf_sum, lds);
     __syncthreads();
@@ -327,14 +414,14 @@ __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, cons
 // update_normal (pcisph.cl:94-112)
 template <typename R, bool FAST>
 __global__ void k_normal(SphParams<R> P, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
-                         V4<R>* __restrict__ nrm) {
+                         V4<R>* __restrict__ nrm, NeighborLists NL) {
     using O = Ops<R, FAST>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const V4<R> self = xr[i];
     const int3 c = cell_of4<R>(self, P.cell);
     R nx = 0, ny = 0, nz = 0;
-    for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+    auto body = [&](uint32_t j) {
         const V4<R> o = xr[j];
         R dx, dy, dz;
         const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
@@ -343,7 +430,9 @@ __global__ void k_normal(SphParams<R> P, const uint2* __restrict__ fcells, const
         nx += O::div(w * dx, o.w);
         ny += O::div(w * dy, o.w);
         nz += O::div(w * dz, o.w);
-    });
+    };
+    const uint32_t len = NL.r2 > 0.f ? NL.fcount[i] : kListOverflow;
+    for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, body);
     V4<R> out;
     out.x = nx * P.normal_scale;
     out.y = ny * P.normal_scale;
@@ -376,95 +465,106 @@ template <typename R, bool FAST>
 __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                              const V4<R>* __restrict__ vel, const V4<R>* __restrict__ nrm,
                              V4<R>* __restrict__ fo, R* __restrict__ p, V4<R>* __restrict__ fp,
-                             V4<R>* __restrict__ xp, FeatureArgs F) {
+                             V4<R>* __restrict__ xp, FeatureArgs F, NeighborLists NL, LoopState* __restrict__ st) {
     using O = Ops<R, FAST>;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    const V4<R> self = xr[i];
-    const V4<R> sv = vel[i];
-    const V4<R> sn = nrm[i];
-    const int3 c = cell_of4<R>(self, P.cell);
-    R vx = 0, vy = 0, vz = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
-    float a_rho = 0.f, a_p = 0.f, a_div = 0.f, a_n = 0.f;
-    const bool want = F.feat != nullptr;
-    for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
-        const V4<R> o = xr[j];
-        R dx, dy, dz;
-        const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
-        if (r2 > P.h2) return;  // every term below vanishes for dist >= h
-        const R dist = O::sqrt(r2);
-        const V4<R> ov = vel[j];
-        if (want) {
-            a_n += 1.f;
-            a_rho += (float)o.w;
-            a_p += (float)ov.w;
-            if (dist > R(0.0001f)) {
-                const float dr = (float)(P.h - dist);
-                a_div += (float)(((ov.x - sv.x) * dx + (ov.y - sv.y) * dy + (ov.z - sv.z) * dz) / dist) * dr * dr /
-                         (float)o.w;
+    float moved = 0.f;
+    if (i < P.n) {
+        const V4<R> self = xr[i];
+        const V4<R> sv = vel[i];
+        const V4<R> sn = nrm[i];
+        const int3 c = cell_of4<R>(self, P.cell);
+        R vx = 0, vy = 0, vz = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
+        float a_rho = 0.f, a_p = 0.f, a_div = 0.f, a_n = 0.f;
+        const bool want = F.feat != nullptr;
+        auto body = [&](uint32_t j) {
+            const V4<R> o = xr[j];
+            R dx, dy, dz;
+            const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+            if (r2 > P.h2) return;  // every term below vanishes for dist >= h
+            const R dist = O::sqrt(r2);
+            const V4<R> ov = vel[j];
+            if (want) {
+                a_n += 1.f;
+                a_rho += (float)o.w;
+                a_p += (float)ov.w;
+                if (dist > R(0.0001f)) {
+                    const float dr = (float)(P.h - dist);
+                    a_div += (float)(((ov.x - sv.x) * dx + (ov.y - sv.y) * dy + (ov.z - sv.z) * dz) / dist) *
+                             dr * dr / (float)o.w;
+                }
             }
+            if (o.w > R(0.0001f)) {
+                const R k = O::div(dist > P.h ? R(0) : P.h - dist, o.w);
+                vx += (ov.x - sv.x) * k;
+                vy += (ov.y - sv.y) * k;
+                vz += (ov.z - sv.z) * k;
+            }
+            if (dist > R(0.0001f) && dist < P.h) {
+                const V4<R> on = nrm[j];
+                const R corr = O::div(R(2) * P.rho0, self.w + o.w);
+                const R stk = surface_tension_kernel<R>(dist, P.h, P.st_term);
+                const R cs = corr * stk;
+                cx += cs * O::div(dx, dist);
+                cy += cs * O::div(dy, dist);
+                cz += cs * O::div(dz, dist);
+                ux += corr * (sn.x - on.x);
+                uy += corr * (sn.y - on.y);
+                uz += corr * (sn.z - on.z);
+            }
+        };
+        const uint32_t len = NL.r2 > 0.f ? NL.fcount[i] : kListOverflow;
+        for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, body);
+        const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
+        const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
+        const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
+        V4<R> f;
+        f.x = ox;
+        f.y = oy;
+        f.z = oz;
+        f.w = 0;
+        fo[i] = f;
+        p[i] = 0;
+        V4<R> z;
+        z.x = z.y = z.z = z.w = 0;
+        fp[i] = z;
+        V4<R> xo, vo;
+        integrate<R, FAST>(P, self, sv, ox + R(0), oy + R(0), oz + R(0), xo, vo);
+        xp[i] = xo;
+        moved = displacement<R>(xo, self);
+        if (want) {
+            const float rho0 = (float)P.rho0;
+            const uint4 cnt = F.counts[i];
+            const float inv = a_n > 0.f ? 1.f / a_n : 0.f;
+            const float speed = sqrtf((float)(sv.x * sv.x + sv.y * sv.y + sv.z * sv.z));
+            float4* out = reinterpret_cast<float4*>(F.feat + (size_t)i * 8);
+            out[0] = make_float4(((float)self.w / rho0 - 1.f) * 100.f, (float)cnt.z / 32.f, (float)sv.w / F.p_ref,
+                                 speed * (float)P.dt / (float)P.h * 100.f);
+            out[1] = make_float4((a_rho * inv / rho0 - 1.f) * 100.f, a_p * inv / F.p_ref, (float)cnt.w / 32.f,
+                                 a_div * F.div_scale);
         }
-        if (o.w > R(0.0001f)) {
-            const R k = O::div(dist > P.h ? R(0) : P.h - dist, o.w);
-            vx += (ov.x - sv.x) * k;
-            vy += (ov.y - sv.y) * k;
-            vz += (ov.z - sv.z) * k;
-        }
-        if (dist > R(0.0001f) && dist < P.h) {
-            const V4<R> on = nrm[j];
-            const R corr = O::div(R(2) * P.rho0, self.w + o.w);
-            const R stk = surface_tension_kernel<R>(dist, P.h, P.st_term);
-            const R cs = corr * stk;
-            cx += cs * O::div(dx, dist);
-            cy += cs * O::div(dy, dist);
-            cz += cs * O::div(dz, dist);
-            ux += corr * (sn.x - on.x);
-            uy += corr * (sn.y - on.y);
-            uz += corr * (sn.z - on.z);
-        }
-    });
-    const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
-    const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
-    const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
-    V4<R> f;
-    f.x = ox;
-    f.y = oy;
-    f.z = oz;
-    f.w = 0;
-    fo[i] = f;
-    p[i] = 0;
-    V4<R> z;
-    z.x = z.y = z.z = z.w = 0;
-    fp[i] = z;
-    V4<R> xo, vo;
-    integrate<R, FAST>(P, self, sv, ox + R(0), oy + R(0), oz + R(0), xo, vo);
-    xp[i] = xo;
-    if (want) {
-        const float rho0 = (float)P.rho0;
-        const uint4 cnt = F.counts[i];
-        const float inv = a_n > 0.f ? 1.f / a_n : 0.f;
-        const float speed = sqrtf((float)(sv.x * sv.x + sv.y * sv.y + sv.z * sv.z));
-        float4* out = reinterpret_cast<float4*>(F.feat + (size_t)i * 8);
-        out[0] = make_float4(((float)self.w / rho0 - 1.f) * 100.f, (float)cnt.z / 32.f, (float)sv.w / F.p_ref,
-                             speed * (float)P.dt / (float)P.h * 100.f);
-        out[1] = make_float4((a_rho * inv / rho0 - 1.f) * 100.f, a_p * inv / F.p_ref, (float)cnt.w / 32.f,
-                             a_div * F.div_scale);
     }
+    wave_max_to(&st->max_disp[0], moved);
 }
 
 // update_pressure (pcisph.cl:212-263): fluid mode for threads 0..n-1, boundary mode
 // for threads n..n+nb-1 (the two launches of Fluid.cpp:263-280 are independent).
 // Also publishes x + p/rho^2 for the force pass and the per-iteration max dv.
+// Distances are taken at the predicted positions, so the step's lists (built at
+// x with radius (1+skin)h) hold every pair within h only while no predicted
+// position has moved more than skin*h/2 from x; past NL.disp_limit the pass walks
+// the buckets instead.
 template <typename R, bool FAST>
 __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict__ st,
                            const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
                            const R* __restrict__ binit, const uint2* __restrict__ fcells,
                            const V4<R>* __restrict__ xr, const V4<R>* __restrict__ xp, R* __restrict__ dv,
                            R* __restrict__ p, V4<R>* __restrict__ xf, R* __restrict__ bp,
-                           V4<R>* __restrict__ bxf) {
+                           V4<R>* __restrict__ bxf, NeighborLists NL) {
     using O = Ops<R, FAST>;
     __shared__ unsigned long long lds[16];
     if (iteration_skipped<R>(st, iter, P)) return;
+    const bool lists = NL.r2 > 0.f && __uint_as_float(st->max_disp[iter]) <= NL.disp_limit;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long my_max = 0, my_exp = 0;
     if (i < P.n) {
@@ -472,13 +572,16 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
         const V4<R> pr = xp[i];
         const int3 c = cell_of4<R>(self, P.cell);
         R pd = 0;
-        if (P.nb)
-            for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
+        if (P.nb) {
+            const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
+            for_neighbours<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) {
                 const V4<R> o = bpos[j];
                 R dx, dy, dz;
                 pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
             });
-        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+        }
+        const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
+        for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) {
             const V4<R> o = xp[j];
             R dx, dy, dz;
             pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
@@ -502,11 +605,14 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
         const int3 c = cell_of4<R>(self, P.cell);
         R pd = 0;
         pd += binit[b];
-        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
+        const uint32_t len = lists ? NL.bfcount[b] : kListOverflow;
+        auto body = [&](uint32_t j) {
             const V4<R> o = xp[j];
             R dx, dy, dz;
             pd += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-        });
+        };
+        if (len != kListOverflow) for_each_listed(NL.bf, len, P.nb, b, body);
+        else for_each_candidate<R>(fcells, c, P.grid, body);
         pd *= P.density_scale;
         const R v = fmax(R(0), pd - P.rho0);
         if (v != R(0)) {
@@ -551,7 +657,8 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
                                  const V4<R>* __restrict__ fo, const V4<R>* __restrict__ vel,
                                  V4<R>* __restrict__ fp, V4<R>* __restrict__ xp, V4<R>* __restrict__ pos_out,
                                  V4<R>* __restrict__ vel_out, const R* __restrict__ p, HostMirror* host,
-                                 unsigned int tag, int warm_pass, V4<R>* __restrict__ xp_save) {
+                                 unsigned int tag, int warm_pass, V4<R>* __restrict__ xp_save,
+                                 NeighborLists NL) {
     using O = Ops<R, FAST>;
     // warm_pass: the pressure force of the network's initial pressures, before the
     // first prediction -- not an iteration of the loop.
@@ -569,48 +676,58 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
         __threadfence_system();
         __hip_atomic_store(&host->tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    if (i >= P.n) return;
     const bool last = !warm_pass && iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations,
                                                          P.rho0, P.threshold);
-    const V4<R> self = xf[i];
-    const R sf = self.w;
-    const int3 c = cell_of4<R>(self, P.cell);
-    R fx = 0, fy = 0, fz = 0;
-    auto add = [&](const V4<R>& o) {
-        R dx, dy, dz;
-        const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
-        if (r2 > P.h2) return;
-        const R r = O::sqrt(r2);
-        if (r > P.h || r < R(0.0001f)) return;  // kernel_spiky_d1 (pcisph.cl:28-37)
-        const R dr = P.h - r;
-        const R factor = sf + o.w;
-        fx += O::div(dx * dr * dr, r) * factor;
-        fy += O::div(dy * dr * dr, r) * factor;
-        fz += O::div(dz * dr * dr, r) * factor;
-    };
-    if (P.nb) for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) { add(bxf[j]); });
-    for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) { add(xf[j]); });
-    V4<R> f;
-    f.x = fx * P.pf_scale;
-    f.y = fy * P.pf_scale;
-    f.z = fz * P.pf_scale;
-    f.w = 0;
-    fp[i] = f;
-    const V4<R> o = fo[i];
-    const V4<R> v = vel[i];
-    V4<R> xo, vo;
-    integrate<R, FAST>(P, self, v, o.x + f.x, o.y + f.y, o.z + f.z, xo, vo);
-    if (!last) {
-        if (warm_pass) xp_save[i] = xp[i];  // keep the plain start's prediction
-        xp[i] = xo;
-    } else {
-        pos_out[i] = xo;
-        vo.w = p[i];  // the converged pressure rides along to the next step
-        vel_out[i] = vo;
-        const bool bad = !(isfinite(xo.x) && isfinite(xo.y) && isfinite(xo.z) && isfinite(vo.x) &&
-                           isfinite(vo.y) && isfinite(vo.z));
-        if (bad) atomicOr(&st->nonfinite, 1u);
+    float moved = 0.f;
+    if (i < P.n) {
+        const V4<R> self = xf[i];
+        const R sf = self.w;
+        const int3 c = cell_of4<R>(self, P.cell);
+        R fx = 0, fy = 0, fz = 0;
+        auto add = [&](const V4<R>& o) {
+            R dx, dy, dz;
+            const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+            if (r2 > P.h2) return;
+            const R r = O::sqrt(r2);
+            if (r > P.h || r < R(0.0001f)) return;  // kernel_spiky_d1 (pcisph.cl:28-37)
+            const R dr = P.h - r;
+            const R factor = sf + o.w;
+            fx += O::div(dx * dr * dr, r) * factor;
+            fy += O::div(dy * dr * dr, r) * factor;
+            fz += O::div(dz * dr * dr, r) * factor;
+        };
+        // positions here are x, where the lists are exact
+        const bool lists = NL.r2 > 0.f;
+        if (P.nb) {
+            const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
+            for_neighbours<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) { add(bxf[j]); });
+        }
+        const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
+        for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { add(xf[j]); });
+        V4<R> f;
+        f.x = fx * P.pf_scale;
+        f.y = fy * P.pf_scale;
+        f.z = fz * P.pf_scale;
+        f.w = 0;
+        fp[i] = f;
+        const V4<R> o = fo[i];
+        const V4<R> v = vel[i];
+        V4<R> xo, vo;
+        integrate<R, FAST>(P, self, v, o.x + f.x, o.y + f.y, o.z + f.z, xo, vo);
+        if (!last) {
+            if (warm_pass) xp_save[i] = xp[i];  // keep the plain start's prediction
+            xp[i] = xo;
+            moved = displacement<R>(xo, self);
+        } else {
+            pos_out[i] = xo;
+            vo.w = p[i];  // the converged pressure rides along to the next step
+            vel_out[i] = vo;
+            const bool bad = !(isfinite(xo.x) && isfinite(xo.y) && isfinite(xo.z) && isfinite(vo.x) &&
+                               isfinite(vo.y) && isfinite(vo.z));
+            if (bad) atomicOr(&st->nonfinite, 1u);
+        }
     }
+    if (!last) wave_max_to(&st->max_disp[warm_pass ? 0 : iter + 1], moved);
 }
 
 // First iteration of a warm-started step: the pressure update of pcisph.cl:212-263

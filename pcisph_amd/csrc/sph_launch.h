@@ -42,6 +42,7 @@ struct DeviceArrays {
     uint32_t* bvals[2] = {nullptr, nullptr};
     uint2* bcells = nullptr;
     LoopState* state = nullptr;
+    NeighborLists NL;  // r2 == 0: no lists this step
 };
 
 struct Launch {

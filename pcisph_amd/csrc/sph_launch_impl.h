@@ -12,20 +12,20 @@ template <typename R, bool FAST>
 void SphLaunch<R, FAST>::density(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A) {
     const uint32_t total = P.n + P.nb;
     hipLaunchKernelGGL((k_density<R, FAST>), dim3(grid_for(total, L.block)), dim3(L.block), 0, L.stream, P,
-                       A.bcells, A.bpos, A.cells, A.pos, A.xr, A.counts, A.bp, A.bxf, A.state);
+                       A.bcells, A.bpos, A.cells, A.pos, A.xr, A.counts, A.bp, A.bxf, A.state, A.NL);
 }
 
 template <typename R, bool FAST>
 void SphLaunch<R, FAST>::normal(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A) {
     hipLaunchKernelGGL((k_normal<R, FAST>), dim3(grid_for(P.n, L.block)), dim3(L.block), 0, L.stream, P,
-                       A.cells, A.xr, A.nrm);
+                       A.cells, A.xr, A.nrm, A.NL);
 }
 
 template <typename R, bool FAST>
 void SphLaunch<R, FAST>::force_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
                                     const FeatureArgs& F) {
     hipLaunchKernelGGL((k_force_init<R, FAST>), dim3(grid_for(P.n, L.block)), dim3(L.block), 0, L.stream,
-                       P, A.cells, A.xr, A.vel, A.nrm, A.fo, A.p, A.fp, A.xp, F);
+                       P, A.cells, A.xr, A.vel, A.nrm, A.fo, A.p, A.fp, A.xp, F, A.NL, A.state);
 }
 
 template <typename R, bool FAST>
@@ -34,7 +34,7 @@ void SphLaunch<R, FAST>::pressure(const Launch& L, const SphParams<R>& P, const 
     const uint32_t total = P.n + P.nb;
     hipLaunchKernelGGL((k_pressure<R, FAST>), dim3(grid_for(total, L.block)), dim3(L.block), 0, L.stream, P,
                        iter, A.state, A.bcells, A.bpos, A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp,
-                       A.bxf);
+                       A.bxf, A.NL);
 }
 
 template <typename R, bool FAST>
@@ -43,7 +43,7 @@ void SphLaunch<R, FAST>::pressure_force(const Launch& L, const SphParams<R>& P, 
                                         V4<R>* xp_save) {
     hipLaunchKernelGGL((k_pressure_force<R, FAST>), dim3(grid_for(P.n, L.block)), dim3(L.block), 0,
                        L.stream, P, iter, A.state, A.bcells, A.bxf, A.cells, A.xf, A.fo, A.vel, A.fp, A.xp,
-                       A.pos_in, A.vel_in, A.p, host, tag, warm_pass, xp_save);
+                       A.pos_in, A.vel_in, A.p, host, tag, warm_pass, xp_save, A.NL);
 }
 
 template <typename R, bool FAST>

@@ -155,3 +155,36 @@ def test_sample_collection(gpu):
     assert x.shape == (3 * 128, 8) and np.all(np.isfinite(x)) and np.all(t >= 0)
     # feature 1 (fluid neighbours / 32) and 6 (boundary neighbours / 32) are counts
     assert np.allclose(x[:, 1] * 32, np.round(x[:, 1] * 32)) and np.allclose(x[:, 6] * 32, np.round(x[:, 6] * 32))
+
+
+@pytest.mark.parametrize("skin", [0.0, 1e-4, 0.2, 0.6])
+def test_neighbour_lists_are_exact(gpu, oracle_lib, skin):
+    """Neighbour lists off (0), on but invalid for every pressure pass (1e-4: the
+    predicted positions always move further than the skin allows), the default, and
+    a wide skin: the IEEE step is the oracle's, bit for bit."""
+    scene = small_scenes()[1]
+    print(f"case lists skin={skin}", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 2, max_iterations=20)
+    sg = run(g, scene, 2, max_iterations=20, neighbor_skin=skin)
+    assert [s["iterations"] for s in so] == [s["iterations"] for s in sg]
+    for f in FLOAT_FIELDS:
+        np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
+
+
+def test_list_overflow_falls_back_to_buckets(gpu, oracle_lib):
+    """A block packed at a quarter of the rest spacing has ~64x the neighbours of the
+    list capacity allows: those particles walk their buckets, results unchanged."""
+    from pcisph_amd import scenes
+    pts = scenes.lattice(8, 8, 8, 0.005, origin=(0.1, 0.1, 0.1))
+    scene = scenes.Scene(name="packed", fluid=pts.astype(np.float32).ravel(),
+                         boundary=np.zeros(0, np.float32), particle_radius=0.01, gravity=0.0)
+    print("case list overflow", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 1, max_iterations=4)
+    sg = run(g, scene, 1, max_iterations=4)
+    assert so[0]["iterations"] == sg[0]["iterations"]
+    for f in (Field.FLUID_DENSITIES, Field.FLUID_POSITIONS, Field.FLUID_VELOCITIES, Field.FLUID_PRESSURES):
+        np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
