@@ -255,13 +255,6 @@ struct ListWriter {
     __device__ __forceinline__ uint32_t result() const { return len <= cap ? len : kListOverflow; }
 };
 
-template <typename F>
-__device__ __forceinline__ void for_each_listed(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
-                                                uint32_t i, F&& body) {
-    const uint32_t* q = list + i;
-#pragma unroll 2
-    for (uint32_t k = 0; k < len; ++k) body(q[(size_t)k * stride]);
-}
 
 template <typename T>
 __device__ __forceinline__ T block_reduce_sum(T v, T* lds) {
@@ -324,14 +317,66 @@ __device__ __forceinline__ float displacement(const V4<R>& a, const V4<R>& b) {
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-// Walks particle i's list if it has one, else its 27 buckets (same order, same
-// non-zero terms).
-template <typename R, typename F>
-__device__ __forceinline__ void for_neighbours(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
-                                               uint32_t i, const uint2* __restrict__ cells, int3 c,
-                                               const HashGrid& g, F&& body) {
-    if (len != kListOverflow) for_each_listed(list, len, stride, i, body);
-    else for_each_candidate<R>(cells, c, g, body);
+// Batched gathers.  A neighbour walk is split into load(j) -> data and
+// use(data, j): kGatherBatch candidates are loaded before any is used, so each lane
+// keeps that many independent loads in flight instead of one; use() still sees the
+// candidates in order, so sums are unchanged.
+constexpr int kGatherBatch = 4;
+
+template <typename Load, typename Use>
+__device__ __forceinline__ void gather_range(uint32_t start, uint32_t end, Load&& load, Use&& use) {
+    using T = decltype(load(0u));
+    uint32_t j = start;
+    for (; j + kGatherBatch <= end; j += kGatherBatch) {
+        T d[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) d[u] = load(j + u);
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) use(d[u], j + u);
+    }
+    for (; j < end; ++j) use(load(j), j);
+}
+
+// The 27 buckets around cell c, each range gathered in batches.
+template <typename R, typename Load, typename Use>
+__device__ __forceinline__ void walk_buckets(const uint2* __restrict__ cells, int3 c, const HashGrid& g, Load&& load,
+                                             Use&& use) {
+    for_each_bucket(c, g, [&](uint32_t key) {
+        const uint2 r = cells[key];
+        gather_range(r.x, r.y, load, use);
+    });
+}
+
+// A slot-major list of `len` entries of particle i.
+template <typename Load, typename Use>
+__device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
+                                          uint32_t i, Load&& load, Use&& use) {
+    using T = decltype(load(0u));
+    const uint32_t* q = list + i;
+    uint32_t k = 0;
+    for (; k + kGatherBatch <= len; k += kGatherBatch) {
+        uint32_t jj[kGatherBatch];
+        T d[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) jj[u] = q[(size_t)(k + u) * stride];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) d[u] = load(jj[u]);
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) use(d[u], jj[u]);
+    }
+    for (; k < len; ++k) {
+        const uint32_t j = q[(size_t)k * stride];
+        use(load(j), j);
+    }
+}
+
+// Particle i's list if it has one, else its 27 buckets (same order, same non-zero terms).
+template <typename R, typename Load, typename Use>
+__device__ __forceinline__ void walk(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride, uint32_t i,
+                                     const uint2* __restrict__ cells, int3 c, const HashGrid& g, Load&& load,
+                                     Use&& use) {
+    if (len != kListOverflow) walk_list(list, len, stride, i, load, use);
+    else walk_buckets<R>(cells, c, g, load, use);
 }
 
 // update_density (pcisph.cl:65-92), plus per-particle candidate / neighbour counts,
@@ -356,24 +401,24 @@ __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, cons
         uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
         ListWriter wb{NL.b, P.n, i, NL.cap_b, 0}, wf{NL.f, P.n, i, NL.cap_f, 0};
         if (P.nb)
-            for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
-                const V4<R> o = bpos[j];
-                R dx, dy, dz;
-                const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
-                rho += poly6<R>(r2, P.h2);
-                ++cb;
-                nb += !(r2 > P.h2);
-                if (lists && r2 <= lr2) wb.push(j);
-            });
-        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
-            const V4<R> o = pos[j];
-            R dx, dy, dz;
-            const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
-            rho += poly6<R>(r2, P.h2);
-            ++cf;
-            nf += !(r2 > P.h2);
-            if (lists && r2 <= lr2) wf.push(j);
-        });
+            walk_buckets<R>(bcells, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                            [&](const V4<R>& o, uint32_t j) {
+                                R dx, dy, dz;
+                                const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+                                rho += poly6<R>(r2, P.h2);
+                                ++cb;
+                                nb += !(r2 > P.h2);
+                                if (lists && r2 <= lr2) wb.push(j);
+                            });
+        walk_buckets<R>(fcells, c, P.grid, [&](uint32_t j) { return pos[j]; },
+                        [&](const V4<R>& o, uint32_t j) {
+                            R dx, dy, dz;
+                            const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+                            rho += poly6<R>(r2, P.h2);
+                            ++cf;
+                            nf += !(r2 > P.h2);
+                            if (lists && r2 <= lr2) wf.push(j);
+                        });
         rho *= P.density_scale;
         V4<R> out = self;
         out.w = rho;
@@ -394,11 +439,11 @@ __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, cons
         if (lists) {
             const int3 c = cell_of4<R>(o, P.cell);
             ListWriter w{NL.bf, P.nb, b, NL.cap_bf, 0};
-            for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
-                const V4<R> q = pos[j];
-                R dx, dy, dz;
-                if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
-            });
+            walk_buckets<R>(fcells, c, P.grid, [&](uint32_t j) { return pos[j]; },
+                            [&](const V4<R>& q, uint32_t j) {
+                                R dx, dy, dz;
+                                if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
+                            });
             NL.bfcount[b] = w.result();
         }
     }
@@ -421,8 +466,7 @@ __global__ void k_normal(SphParams<R> P, const uint2* __restrict__ fcells, const
     const V4<R> self = xr[i];
     const int3 c = cell_of4<R>(self, P.cell);
     R nx = 0, ny = 0, nz = 0;
-    auto body = [&](uint32_t j) {
-        const V4<R> o = xr[j];
+    auto body = [&](const V4<R>& o, uint32_t) {
         R dx, dy, dz;
         const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
         if (r2 > P.h2) return;  // poly6_d1 is exactly zero there (pcisph.cl:20-21)
@@ -432,7 +476,7 @@ __global__ void k_normal(SphParams<R> P, const uint2* __restrict__ fcells, const
         nz += O::div(w * dz, o.w);
     };
     const uint32_t len = NL.r2 > 0.f ? NL.fcount[i] : kListOverflow;
-    for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, body);
+    walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { return xr[j]; }, body);
     V4<R> out;
     out.x = nx * P.normal_scale;
     out.y = ny * P.normal_scale;
@@ -477,13 +521,17 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
         R vx = 0, vy = 0, vz = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
         float a_rho = 0.f, a_p = 0.f, a_div = 0.f, a_n = 0.f;
         const bool want = F.feat != nullptr;
-        auto body = [&](uint32_t j) {
-            const V4<R> o = xr[j];
+        struct Cand {
+            V4<R> o, ov, on;
+        };
+        auto load = [&](uint32_t j) { return Cand{xr[j], vel[j], nrm[j]}; };
+        auto body = [&](const Cand& cd, uint32_t) {
+            const V4<R>& o = cd.o;
             R dx, dy, dz;
             const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
             if (r2 > P.h2) return;  // every term below vanishes for dist >= h
             const R dist = O::sqrt(r2);
-            const V4<R> ov = vel[j];
+            const V4<R>& ov = cd.ov;
             if (want) {
                 a_n += 1.f;
                 a_rho += (float)o.w;
@@ -501,7 +549,7 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
                 vz += (ov.z - sv.z) * k;
             }
             if (dist > R(0.0001f) && dist < P.h) {
-                const V4<R> on = nrm[j];
+                const V4<R>& on = cd.on;
                 const R corr = O::div(R(2) * P.rho0, self.w + o.w);
                 const R stk = surface_tension_kernel<R>(dist, P.h, P.st_term);
                 const R cs = corr * stk;
@@ -514,7 +562,7 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
             }
         };
         const uint32_t len = NL.r2 > 0.f ? NL.fcount[i] : kListOverflow;
-        for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, body);
+        walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, load, body);
         const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
         const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
         const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
@@ -574,18 +622,18 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
         R pd = 0;
         if (P.nb) {
             const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
-            for_neighbours<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) {
-                const V4<R> o = bpos[j];
-                R dx, dy, dz;
-                pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-            });
+            walk<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                    [&](const V4<R>& o, uint32_t) {
+                        R dx, dy, dz;
+                        pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                    });
         }
         const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
-        for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) {
-            const V4<R> o = xp[j];
-            R dx, dy, dz;
-            pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-        });
+        walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { return xp[j]; },
+                [&](const V4<R>& o, uint32_t) {
+                    R dx, dy, dz;
+                    pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                });
         pd *= P.density_scale;
         const R v = fmax(R(0), pd - P.rho0);
         my_exp = Bits<R>::of(fmax(R(0), P.rho0 - pd));
@@ -606,13 +654,11 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
         R pd = 0;
         pd += binit[b];
         const uint32_t len = lists ? NL.bfcount[b] : kListOverflow;
-        auto body = [&](uint32_t j) {
-            const V4<R> o = xp[j];
-            R dx, dy, dz;
-            pd += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-        };
-        if (len != kListOverflow) for_each_listed(NL.bf, len, P.nb, b, body);
-        else for_each_candidate<R>(fcells, c, P.grid, body);
+        walk<R>(NL.bf, len, P.nb, b, fcells, c, P.grid, [&](uint32_t j) { return xp[j]; },
+                [&](const V4<R>& o, uint32_t) {
+                    R dx, dy, dz;
+                    pd += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                });
         pd *= P.density_scale;
         const R v = fmax(R(0), pd - P.rho0);
         if (v != R(0)) {
@@ -700,10 +746,12 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
         const bool lists = NL.r2 > 0.f;
         if (P.nb) {
             const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
-            for_neighbours<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) { add(bxf[j]); });
+            walk<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) { return bxf[j]; },
+                    [&](const V4<R>& o, uint32_t) { add(o); });
         }
         const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
-        for_neighbours<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { add(xf[j]); });
+        walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { return xf[j]; },
+                [&](const V4<R>& o, uint32_t) { add(o); });
         V4<R> f;
         f.x = fx * P.pf_scale;
         f.y = fy * P.pf_scale;
