@@ -110,9 +110,12 @@ typedef struct pcisph_step_stats {
                                        rejected for the plain start (larger residual) */
     uint64_t fluid_candidates;      /* sum over fluid particles of fluid candidates */
     uint64_t boundary_candidates;   /* sum over fluid particles of boundary candidates */
-    float max_density_expansion;    /* max over fluid of max(0, rho0 - rho*), last iteration:
-                                       the side of the residual the loop does not bound */
-    uint32_t reserved;
+    float max_density_expansion;    /* max of max(0, rho0 - rho*) over fluid particles whose
+                                       step-start density was >= rho0 (free-surface particles
+                                       sit below rho0 regardless), last iteration: the side of
+                                       the residual the loop does not bound */
+    uint32_t list_passes;           /* pressure passes of this step that walked the neighbour
+                                       lists (the others walked the 27 buckets) */
 } pcisph_step_stats;
 
 typedef struct pcisph_solver pcisph_solver;

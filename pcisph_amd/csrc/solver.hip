@@ -222,6 +222,7 @@ struct StepResult {
     uint64_t fluid_candidates = 0, boundary_candidates = 0;
     bool warm = false, warm_rejected = false;
     uint64_t samples = 0;
+    uint32_t list_passes = 0;
 };
 
 // What a step does for the warm-start network: use it (params != null) and/or
@@ -647,6 +648,7 @@ class Engine final : public EngineBase {
         r.boundary_candidates = host_->candidates[1];
         r.warm = warm;
         r.warm_rejected = warm && host_->warm_rejected;
+        r.list_passes = host_->list_passes;
         return r;
     }
 
@@ -1021,6 +1023,7 @@ static void step_impl(pcisph_solver* s, pcisph_step_stats* stats) {
         s->samples_count = std::min(s->samples_cap, s->samples_count + r.samples);
         st.warm_start = (r.warm ? 1u : 0u) | (r.warm_rejected ? 2u : 0u);
         st.max_density_expansion = (float)r.max_expansion;
+        st.list_passes = r.list_passes;
         s->boundary_updated = false;
         st.iterations = r.iterations;
         st.converged = r.converged;

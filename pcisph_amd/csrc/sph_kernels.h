@@ -51,7 +51,8 @@ struct SphParams {
 // density variation of iteration k (integer order = float order for x >= 0).
 struct LoopState {
     unsigned long long max_dv[kMaxIterationsCap];
-    unsigned long long max_exp[kMaxIterationsCap];  // max(0, rho0 - rho*): the signed residual's other side
+    unsigned long long max_exp[kMaxIterationsCap];  // max(0, rho0 - rho*) over particles that began the
+                                                    // step at or above rho0: the other side of the residual
     unsigned int max_disp[kMaxIterationsCap];        // float bits: max |xp - x| entering iteration k
     unsigned long long fluid_candidates;
     unsigned long long boundary_candidates;
@@ -60,7 +61,8 @@ struct LoopState {
     unsigned long long dual_exp[2];
     unsigned int nonfinite;
     unsigned int warm_rejected;  // 1: the plain start had the smaller residual
-    unsigned int pad[2];
+    unsigned int list_passes;    // pressure passes that walked the neighbour lists
+    unsigned int pad;
 };
 
 // Second set of per-particle arrays for the dual first iteration of a warm start:
@@ -82,6 +84,7 @@ struct HostMirror {
     unsigned long long max_exp[kMaxIterationsCap];
     unsigned long long candidates[2];
     unsigned int warm_rejected;
+    unsigned int list_passes;
     unsigned int tag[kMaxIterationsCap];
 };
 
@@ -614,6 +617,7 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
     if (iteration_skipped<R>(st, iter, P)) return;
     const bool lists = NL.r2 > 0.f && __uint_as_float(st->max_disp[iter]) <= NL.disp_limit;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && lists) atomicAdd(&st->list_passes, 1u);
     unsigned long long my_max = 0, my_exp = 0;
     if (i < P.n) {
         const V4<R> self = xr[i];
@@ -636,7 +640,9 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
                 });
         pd *= P.density_scale;
         const R v = fmax(R(0), pd - P.rho0);
-        my_exp = Bits<R>::of(fmax(R(0), P.rho0 - pd));
+        // expansion counts only where the step began at or above rest density: free-
+        // surface particles sit below rho0 whatever the pressure
+        if (self.w >= P.rho0) my_exp = Bits<R>::of(fmax(R(0), P.rho0 - pd));
         dv[i] = v;
         R pi = p[i];
         if (v != R(0)) {
@@ -719,6 +725,7 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
         host->candidates[0] = st->fluid_candidates;
         host->candidates[1] = st->boundary_candidates;
         host->warm_rejected = st->warm_rejected;
+        host->list_passes = st->list_passes;
         __threadfence_system();
         __hip_atomic_store(&host->tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
@@ -816,9 +823,10 @@ __global__ void k_pressure_dual(SphParams<R> P, LoopState* __restrict__ st, cons
         sz *= P.density_scale;
         const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
         m[0] = Bits<R>::of(vw);
-        m[1] = Bits<R>::of(fmax(R(0), P.rho0 - sw));
+        const bool bulk = self.w >= P.rho0;  // as in k_pressure: no free-surface particles
+        m[1] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sw)) : 0ull;
         m[2] = Bits<R>::of(vz);
-        m[3] = Bits<R>::of(fmax(R(0), P.rho0 - sz));
+        m[3] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sz)) : 0ull;
         dv[i] = vw;
         D.dv0[i] = vz;
         R pi = p[i];
