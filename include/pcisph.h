@@ -253,10 +253,34 @@ int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float lea
                      float* last_loss);
 int pcisph_set_warm_start(pcisph_solver* s, int enable);
 
+/* ---- batches of scenes (no counterpart in the reference) -------------------------
+ * A handle holds `count` independent scenes (default 1), each what one sim::Fluid
+ * holds (Fluid.h:13-102): its own parameters, particle counts, threshold and
+ * buffers.  The setters, pcisph_get_params, pcisph_set_density_variation_threshold,
+ * pcisph_field_size, the uploads and downloads and pcisph_translate_boundary address
+ * the selected scene (scene 0 until pcisph_select_scene); solver options, precision
+ * and the warm-start network are shared.  One step advances every scene with one
+ * launch per stage; each scene's correction loop ends on its own test, and the step
+ * ends when the last scene's loop ends.  Sort products downloaded per scene are in
+ * the scene's own numbering (keys < its bucket count, indices < its counts), exactly
+ * as a single-scene handle gives them.  Changing the count marks every scene's
+ * parameters changed (velocities zeroed at the next step, as after any setter);
+ * positions and velocities of the scenes that remain are kept. */
+int pcisph_set_scene_count(pcisph_solver* s, uint32_t count);
+int pcisph_get_scene_count(const pcisph_solver* s, uint32_t* count);
+int pcisph_select_scene(pcisph_solver* s, uint32_t index);
+/* One step of every scene with per-scene statistics (count = scene count).
+ * pcisph_step / pcisph_run report the batch as one: the largest iteration count and
+ * error, converged only if every scene converged, candidates summed. */
+int pcisph_step_scenes(pcisph_solver* s, pcisph_step_stats* stats, size_t count);
+
 /* Rigid motion of the walls (moving tanks): adds (dx, dy, dz) to every boundary
  * position on the device and re-sorts the boundary at the next step.  No counterpart
  * in the reference, whose boundary never moves. */
 int pcisph_translate_boundary(pcisph_solver* s, float dx, float dy, float dz);
+/* Every scene's walls in one launch: scene k moves by shifts[3k .. 3k+2]
+ * (count = 3 * scene count); scenes with a non-zero shift are re-sorted next step. */
+int pcisph_translate_boundaries(pcisph_solver* s, const float* shifts, size_t count);
 
 /* Autotuner: times `trial_steps` steps of the current scene for each threads-per-block
  * choice (64..512) and radix digit width (5..8 bits) from the same saved starting state,

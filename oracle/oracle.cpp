@@ -117,6 +117,10 @@ void derive_params(Simulation_Params& p) {
     p.surface_tension_normalization = 32.f / (kPi * std::pow(p.kernel_radius, 9.f));
     p.surface_tension_term = std::pow(p.kernel_radius, 6.f) / 64.f;
     const float particle_size = 2.f * p.particle_radius;
+    if (particle_size == 0.f) {  // the lattice loop would not advance (the reference hangs)
+        p.density_variation_scaling_factor = 0.f;
+        return;
+    }
     const float beta = p.delta_t * p.delta_t * p.particle_mass * p.particle_mass * 2.f /
                        (p.rest_density * p.rest_density);
     float sum[3] = {0.f, 0.f, 0.f};

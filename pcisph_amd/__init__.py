@@ -37,6 +37,8 @@ EXPORTS = (
     "pcisph_samples_reserve", "pcisph_samples_collect", "pcisph_samples_count",
     "pcisph_samples_download", "pcisph_samples_upload", "pcisph_mlp_train", "pcisph_set_warm_start",
     "pcisph_autotune", "pcisph_translate_boundary",
+    "pcisph_set_scene_count", "pcisph_get_scene_count", "pcisph_select_scene", "pcisph_step_scenes",
+    "pcisph_translate_boundaries",
 )
 
 MLP_INPUTS = 8
@@ -73,6 +75,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.pcisph_samples_collect.argtypes = [C.c_void_p, C.c_uint32]
     L.pcisph_set_warm_start.argtypes = [C.c_void_p, C.c_int]
     L.pcisph_translate_boundary.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
+    L.pcisph_set_scene_count.argtypes = [C.c_void_p, C.c_uint32]
+    L.pcisph_select_scene.argtypes = [C.c_void_p, C.c_uint32]
+    L.pcisph_step_scenes.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    L.pcisph_translate_boundaries.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     _lib = L
     return L
 
@@ -125,6 +131,34 @@ class Fluid(SolverBase):
 
     def sync(self):
         self._call("sync")
+
+    # --- batches of scenes
+    def set_scene_count(self, count: int):
+        self._call("set_scene_count", C.c_uint32(int(count)))
+
+    def scene_count(self) -> int:
+        n = C.c_uint32()
+        self._call("get_scene_count", C.byref(n))
+        return n.value
+
+    def select_scene(self, index: int):
+        """Setters, uploads and downloads address scene `index` from now on."""
+        self._call("select_scene", C.c_uint32(int(index)))
+
+    def step_scenes(self) -> list[dict]:
+        """One step of every scene; per-scene stats."""
+        n = self.scene_count()
+        arr = (StepStats * n)()
+        self._call("step_scenes", C.cast(arr, C.c_void_p), C.c_size_t(n))
+        return [s.as_dict() for s in arr]
+
+    def load_scenes(self, scenes, min_iterations=None, max_iterations=None, **options):
+        """A batch: scene k of `scenes` loaded (as load_scene does) into scene k."""
+        self.set_scene_count(len(scenes))
+        for k, sc in enumerate(scenes):
+            self.select_scene(k)
+            self.load_scene(sc, min_iterations=min_iterations, max_iterations=max_iterations, **options)
+        self.select_scene(0)
 
     def prepare(self):
         """Apply pending setter changes now (see pcisph_prepare)."""
@@ -204,6 +238,11 @@ class Fluid(SolverBase):
     def translate_boundary(self, dx: float, dy: float, dz: float):
         """Move the walls rigidly by (dx, dy, dz); the boundary is re-sorted next step."""
         self._call("translate_boundary", dx, dy, dz)
+
+    def translate_boundaries(self, shifts):
+        """Move every scene's walls at once: shifts[k] = (dx, dy, dz) of scene k."""
+        a = np.ascontiguousarray(shifts, np.float32).ravel()
+        self._call("translate_boundaries", a.ctypes.data_as(C.c_void_p), C.c_size_t(a.size))
 
     def set_warm_start(self, on: bool = True):
         self._call("set_warm_start", C.c_int(1 if on else 0))

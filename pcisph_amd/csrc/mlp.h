@@ -33,11 +33,10 @@ void mlp_gradients(const MlpLaunch& L, const float* params, const float* x, cons
 void mlp_adam(const MlpLaunch& L, float* params, const float* grad, float* m, float* v, uint32_t step, float lr,
               float beta1, float beta2, float eps);
 
-// Warm start: p[i] = max(0, net(feat[i])) * p_ref, and xf[i] = (x_i, p[i] / rho_i^2)
-// from xr[i] = (x_i, rho_i).
-template <typename R, typename V>
-void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, uint32_t n, float p_ref,
-                    const V* xr, R* p, V* xf);
+// Warm start: y[i] = max(0, net(feat[i])), the initial pressure in units of the
+// scene's p_ref.
+template <typename R>
+void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, uint32_t n, R* y);
 // Appends (feat[i], p[i] / p_ref) for i = phase, phase + stride, ... to the sample
 // store at slots base + i / stride (those below cap).
 template <typename R>

@@ -316,10 +316,9 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, floa
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-template <typename R, typename V>
+template <typename R>
 __global__ __launch_bounds__(kThreads) void k_mlp_warm(const float* __restrict__ P, const float* __restrict__ x,
-                                                       uint32_t n, float p_ref, const V* __restrict__ xr,
-                                                       R* __restrict__ p, V* __restrict__ xf) {
+                                                       uint32_t n, R* __restrict__ y) {
     __shared__ Weights s;
     stage_weights(s, P);
     __syncthreads();
@@ -330,13 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_warm(const float* __restrict__
         const uint32_t row = row0 + (lane & 15);
         v4f z1[4], z2[4];
         const float out = tile_forward(s, load_x(x, row, n, g), z1, z2);
-        if (g == 0 && row < n) {
-            const R pi = (R)(fmaxf(out, 0.f) * p_ref);
-            p[row] = pi;
-            V o = xr[row];
-            o.w = pi / (o.w * o.w);
-            xf[row] = o;
-        }
+        if (g == 0 && row < n) y[row] = (R)fmaxf(out, 0.f);
     }
 }
 
@@ -396,17 +389,13 @@ void mlp_gradients(const MlpLaunch& L, const float* params, const float* x, cons
                        loss);
 }
 
-template <typename R, typename V>
-void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, uint32_t n, float p_ref,
-                    const V* xr, R* p, V* xf) {
+template <typename R>
+void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, uint32_t n, R* y) {
     if (!n) return;
-    hipLaunchKernelGGL((k_mlp_warm<R, V>), dim3(blocks_for(n)), dim3(kThreads), 0, L.stream, params, feat, n, p_ref,
-                       xr, p, xf);
+    hipLaunchKernelGGL((k_mlp_warm<R>), dim3(blocks_for(n)), dim3(kThreads), 0, L.stream, params, feat, n, y);
 }
-template void mlp_warm_start<float, float4>(const MlpLaunch&, const float*, const float*, uint32_t, float,
-                                            const float4*, float*, float4*);
-template void mlp_warm_start<double, double4>(const MlpLaunch&, const float*, const float*, uint32_t, float,
-                                              const double4*, double*, double4*);
+template void mlp_warm_start<float>(const MlpLaunch&, const float*, const float*, uint32_t, float*);
+template void mlp_warm_start<double>(const MlpLaunch&, const float*, const float*, uint32_t, double*);
 
 template <typename R>
 void mlp_collect(const MlpLaunch& L, const float* feat, const R* p, uint32_t n, uint32_t stride, uint32_t phase,

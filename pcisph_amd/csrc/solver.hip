@@ -62,6 +62,10 @@ void derive_params(Simulation_Params& p) {
     // PCISPH scaling factor delta from a prototype particle with a full lattice
     // neighbourhood at spacing 2r (Solenthaler & Pajarola 2009, eq. 8).
     const float spacing = 2.f * p.particle_radius;
+    if (spacing == 0.f) {  // the lattice loop below would not advance (the reference hangs)
+        p.density_variation_scaling_factor = 0.f;
+        return;
+    }
     const float beta = p.delta_t * p.delta_t * p.particle_mass * p.particle_mass * 2.f /
                        (p.rest_density * p.rest_density);
     float grad_sum[3] = {0.f, 0.f, 0.f};
@@ -88,6 +92,7 @@ void derive_params(Simulation_Params& p) {
 template <typename R>
 SphParams<R> make_sph_params(const Simulation_Params& p, float threshold, const pcisph_solver_options& o) {
     SphParams<R> P;
+    std::memset(&P, 0, sizeof(P));  // padding too: scene tables are compared bytewise
     P.h = (R)p.kernel_radius;
     P.h2 = (R)p.kernel_radius2;
     P.cell = (R)p.cell_size;
@@ -221,12 +226,12 @@ struct StepResult {
     double max_dv = 0, max_expansion = 0;
     uint64_t fluid_candidates = 0, boundary_candidates = 0;
     bool warm = false, warm_rejected = false;
-    uint64_t samples = 0;
     uint32_t list_passes = 0;
 };
 
 // What a step does for the warm-start network: use it (params != null) and/or
-// append (features, converged pressure) samples of every stride-th particle.
+// append (features, converged pressure) samples of every stride-th particle of
+// every scene.
 struct WarmArgs {
     const float* params = nullptr;
     uint32_t collect_stride = 0, collect_phase = 0;
@@ -235,23 +240,35 @@ struct WarmArgs {
     uint64_t store_base = 0, store_cap = 0;
 };
 
+// One scene of a batch as the engine sees it: its parameters (derived), the
+// density-variation threshold of its loop.
+struct SceneInput {
+    const Simulation_Params* params;
+    float threshold;
+};
+
 class EngineBase {
    public:
     virtual ~EngineBase() = default;
-    virtual void resize_public(uint32_t n, uint32_t nb) = 0;
-    virtual void realloc_internal(const Simulation_Params& p) = 0;
-    virtual void zero_velocities() = 0;
-    virtual void upload(int field, const void* src, size_t count, bool device) = 0;
-    virtual void download(int field, void* dst, size_t count, bool device, bool f64) = 0;
-    virtual StepResult step(const Simulation_Params& p, float threshold, const pcisph_solver_options& o,
-                            bool sort_boundary, bool fast, const WarmArgs& w) = 0;
-    virtual uint32_t nonfinite() = 0;
-    virtual size_t field_size(int field, const Simulation_Params& p) const = 0;
+    // fluid / boundary counts of every scene; keeps each scene's positions,
+    // velocities and boundary positions (up to the smaller of old and new count)
+    virtual void set_counts(const std::vector<uint32_t>& n, const std::vector<uint32_t>& nb) = 0;
+    // bucket tables and sort scratch for the scenes' bucket counts
+    virtual void realloc_internal(const std::vector<SceneInput>& scenes) = 0;
+    virtual void zero_velocities(uint32_t scene) = 0;
+    virtual void upload(uint32_t scene, int field, const void* src, size_t count, bool device) = 0;
+    virtual void download(uint32_t scene, int field, void* dst, size_t count, bool device, bool f64) = 0;
+    virtual void step(const std::vector<SceneInput>& scenes, const pcisph_solver_options& o, bool sort_boundary,
+                      bool fast, const WarmArgs& w, std::vector<StepResult>& out, uint64_t* samples) = 0;
+    virtual std::vector<uint32_t> nonfinite() = 0;
+    virtual size_t field_size(uint32_t scene, int field, const Simulation_Params& p) const = 0;
     virtual void profile_enable(bool on) = 0;
     virtual void profile_read(pcisph_stage_times* out) = 0;
+    virtual void translate_boundary(uint32_t scene, float dx, float dy, float dz) = 0;
+    // every scene's walls at once: shifts[3 s .. 3 s + 2] for scene s
+    virtual void translate_boundaries(const float* shifts, uint32_t block) = 0;
     // device-side copy of the state a step starts from (positions, velocities with the
     // previous pressures, boundary positions), for trial runs of the autotuner
-    virtual void translate_boundary(float dx, float dy, float dz) = 0;
     virtual void snapshot() = 0;
     virtual void restore() = 0;
 };
@@ -295,6 +312,17 @@ void dalloc(T*& p, size_t count, hipStream_t s) {
     HIP_CHECK(hipMemsetAsync((void*)p, 0, bytes, s));
 }
 
+// Exclusive prefix sums of `v` (size + 1 entries).
+std::vector<uint32_t> offsets_of(const std::vector<uint32_t>& v) {
+    std::vector<uint32_t> o(v.size() + 1, 0);
+    for (size_t i = 0; i < v.size(); ++i) {
+        const uint64_t next = (uint64_t)o[i] + v[i];
+        if (next > 0xFFFFFFFFull - 1024) throw Error(PCISPH_ERR_SIZE, "more than 2^32 particles or buckets in a batch");
+        o[i + 1] = (uint32_t)next;
+    }
+    return o;
+}
+
 template <typename R>
 class Engine final : public EngineBase {
    public:
@@ -303,14 +331,12 @@ class Engine final : public EngineBase {
         HIP_CHECK(hipHostMalloc((void**)&host_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
         std::memset((void*)host_, 0, sizeof(HostMirror));
         HIP_CHECK(hipHostGetDevicePointer((void**)&dev_host_, (void*)host_, 0));
-        dalloc(A_.state, 1, stream_);
     }
 
     ~Engine() override {
         (void)hipStreamSynchronize(stream_);
         free_public();
         free_internal();
-        dfree(A_.state);
         dfree(staging_);
         dfree(feat_);
         drop_snapshot();
@@ -326,6 +352,13 @@ class Engine final : public EngineBase {
         dfree(D_.dv0);
         dfree(D_.bp0);
         dfree(D_.bxf0);
+        dfree(scene_table_);
+        dfree(block_table_);
+        dfree(states_);
+        dfree(shift_dev_);
+        if (shift_host_) (void)hipHostFree(shift_host_);
+        if (shift_event_) (void)hipEventDestroy(shift_event_);
+        if (results_) (void)hipHostFree((void*)results_);
         if (host_) (void)hipHostFree((void*)host_);
     }
 
@@ -349,50 +382,95 @@ class Engine final : public EngineBase {
         dfree(sort_scratch_);
     }
 
-    void resize_public(uint32_t n, uint32_t nb) override {
+    // New arrays of `count` elements holding each scene's old contents (at most the
+    // smaller of its old and new count) at the scene's new offset.
+    template <typename T>
+    void regrow(T*& p, uint32_t count, const std::vector<uint32_t>& old_off, const std::vector<uint32_t>& new_off) {
+        T* fresh = nullptr;
+        dalloc(fresh, count, stream_);
+        if (p) {
+            const size_t scenes = std::min(old_off.size(), new_off.size()) - 1;
+            for (size_t s = 0; s < scenes; ++s) {
+                const uint32_t keep = std::min(old_off[s + 1] - old_off[s], new_off[s + 1] - new_off[s]);
+                if (keep)
+                    HIP_CHECK(hipMemcpyAsync(fresh + new_off[s], p + old_off[s], sizeof(T) * keep,
+                                             hipMemcpyDeviceToDevice, stream_));
+            }
+        }
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        dfree(p);
+        p = fresh;
+    }
+
+    void set_counts(const std::vector<uint32_t>& n, const std::vector<uint32_t>& nb) override {
         HIP_CHECK(hipSetDevice(device_));
         HIP_CHECK(hipStreamSynchronize(stream_));
-        if (n != A_.n) {
-            for (auto** q : {&A_.pos_in, &A_.vel_in, &A_.pos, &A_.vel, &A_.xr, &A_.nrm, &A_.fo, &A_.fp, &A_.xp,
-                             &A_.xf})
-                dalloc(*q, n, stream_);
-            dalloc(A_.p, n, stream_);
-            dalloc(A_.dv, n, stream_);
-            dalloc(A_.counts, n, stream_);
+        const std::vector<uint32_t> f_off = offsets_of(n), b_off = offsets_of(nb);
+        const uint32_t N = f_off.back(), Nb = b_off.back();
+        if (f_off != f_off_) {
+            regrow(A_.pos_in, N, f_off_, f_off);
+            regrow(A_.vel_in, N, f_off_, f_off);
+            for (auto** q : {&A_.pos, &A_.vel, &A_.xr, &A_.nrm, &A_.fo, &A_.fp, &A_.xp, &A_.xf}) dalloc(*q, N, stream_);
+            dalloc(A_.p, N, stream_);
+            dalloc(A_.dv, N, stream_);
+            dalloc(A_.counts, N, stream_);
             for (int i = 0; i < 2; i++) {
-                dalloc(A_.keys[i], n, stream_);
-                dalloc(A_.vals[i], n, stream_);
+                dalloc(A_.keys[i], N, stream_);
+                dalloc(A_.vals[i], N, stream_);
             }
-            A_.n = n;
+            A_.n = N;
             fluid_sorted_ = 0;
         }
-        if (nb != A_.nb) {
-            for (auto** q : {&A_.bpos, &A_.bpos_tmp, &A_.bxf}) dalloc(*q, nb, stream_);
-            dalloc(A_.bp, nb, stream_);
-            dalloc(A_.binit, nb, stream_);
+        if (b_off != b_off_) {
+            regrow(A_.bpos, Nb, b_off_, b_off);
+            regrow(A_.binit, Nb, b_off_, b_off);
+            for (auto** q : {&A_.bpos_tmp, &A_.bxf}) dalloc(*q, Nb, stream_);
+            dalloc(A_.bp, Nb, stream_);
             for (int i = 0; i < 2; i++) {
-                dalloc(A_.bkeys[i], nb, stream_);
-                dalloc(A_.bvals[i], nb, stream_);
+                dalloc(A_.bkeys[i], Nb, stream_);
+                dalloc(A_.bvals[i], Nb, stream_);
             }
-            A_.nb = nb;
+            A_.nb = Nb;
             boundary_sorted_ = 0;
         }
-        const size_t stage = std::max<size_t>(4 * (size_t)std::max(n, nb), 1);
+        n_ = n;
+        nb_ = nb;
+        f_off_ = f_off;
+        b_off_ = b_off;
+        uint32_t largest = 1;
+        for (size_t s = 0; s < n.size(); ++s) largest = std::max({largest, n[s], nb[s]});
+        const size_t stage = 4 * (size_t)largest;
         if (stage > staging_words_) {
             dalloc(staging_, stage * 2, stream_);  // room for double results too
             staging_words_ = stage;
         }
+        if (states_n_ != n.size()) {
+            dalloc(states_, std::max<size_t>(n.size(), 1), stream_);
+            if (results_) (void)hipHostFree((void*)results_);
+            results_ = nullptr;
+            HIP_CHECK(hipHostMalloc((void**)&results_, sizeof(SceneResult) * std::max<size_t>(n.size(), 1),
+                                    hipHostMallocMapped | hipHostMallocCoherent));
+            std::memset((void*)results_, 0, sizeof(SceneResult) * std::max<size_t>(n.size(), 1));
+            HIP_CHECK(hipHostGetDevicePointer((void**)&dev_results_, (void*)results_, 0));
+            states_n_ = (uint32_t)n.size();
+        }
+        blocks_for_ = 0;  // block table out of date
         HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
-    void realloc_internal(const Simulation_Params& p) override {
+    void realloc_internal(const std::vector<SceneInput>& scenes) override {
         HIP_CHECK(hipSetDevice(device_));
         HIP_CHECK(hipStreamSynchronize(stream_));
-        if (p.bucket_count != A_.B || !A_.cells) {
-            dalloc(A_.cells, p.bucket_count, stream_);
-            dalloc(A_.bcells, p.bucket_count, stream_);
-            A_.B = p.bucket_count;
+        std::vector<uint32_t> buckets(scenes.size());
+        for (size_t s = 0; s < scenes.size(); ++s) buckets[s] = scenes[s].params->bucket_count;
+        const std::vector<uint32_t> base = offsets_of(buckets);
+        if (base.back() != A_.B || !A_.cells) {
+            dalloc(A_.cells, base.back(), stream_);
+            dalloc(A_.bcells, base.back(), stream_);
+            A_.B = base.back();
         }
+        buckets_ = buckets;
+        bucket_base_ = base;
         const size_t words = std::max(radix_sort_scratch_words(A_.n, 8), radix_sort_scratch_words(A_.nb, 8));
         if (words > sort_scratch_words_) {
             dalloc(sort_scratch_, words, stream_);
@@ -403,12 +481,38 @@ class Engine final : public EngineBase {
         if (A_.nb) HIP_CHECK(hipMemsetAsync(A_.binit, 0, sizeof(R) * A_.nb, stream_));
     }
 
-    void zero_velocities() override {
-        if (A_.n) HIP_CHECK(hipMemsetAsync(A_.vel_in, 0, sizeof(V4<R>) * A_.n, stream_));
+    void zero_velocities(uint32_t scene) override {
+        if (n_[scene])
+            HIP_CHECK(hipMemsetAsync(A_.vel_in + f_off_[scene], 0, sizeof(V4<R>) * n_[scene], stream_));
     }
 
-    void translate_boundary(float dx, float dy, float dz) override {
-        DataLaunch<R>::translate(Launch{stream_, 256}, A_.bpos, A_.nb, dx, dy, dz);
+    void translate_boundary(uint32_t scene, float dx, float dy, float dz) override {
+        DataLaunch<R>::translate(Launch{stream_, 256}, A_.bpos + b_off_[scene], nb_[scene], dx, dy, dz);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    void translate_boundaries(const float* shifts, uint32_t block) override {
+        prepare_blocks(block);
+        const size_t S = n_.size();
+        if (shift_cap_ < S) {
+            HIP_CHECK(hipStreamSynchronize(stream_));
+            if (shift_host_) (void)hipHostFree(shift_host_);
+            shift_host_ = nullptr;
+            HIP_CHECK(hipHostMalloc((void**)&shift_host_, sizeof(float) * 3 * S, hipHostMallocDefault));
+            dalloc(shift_dev_, 3 * S, stream_);
+            if (!shift_event_) HIP_CHECK(hipEventCreateWithFlags(&shift_event_, hipEventDisableTiming));
+            shift_cap_ = S;
+        } else {
+            HIP_CHECK(hipEventSynchronize(shift_event_));  // the last copy out of shift_host_ is done
+        }
+        std::memcpy(shift_host_, shifts, sizeof(float) * 3 * S);
+        HIP_CHECK(hipMemcpyAsync(shift_dev_, shift_host_, sizeof(float) * 3 * S, hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipEventRecord(shift_event_, stream_));
+        Batch<R> B{};
+        B.blocks = block_table_;
+        B.fluid_blocks = fluid_blocks_;
+        B.boundary_blocks = boundary_blocks_;
+        DataLaunch<R>::translate_scenes(Launch{stream_, block}, B, shift_dev_, A_.bpos);
         HIP_CHECK(hipGetLastError());
     }
 
@@ -436,7 +540,7 @@ class Engine final : public EngineBase {
         dfree(snap_b_);
     }
 
-    size_t field_size(int field, const Simulation_Params& p) const override {
+    size_t field_size(uint32_t, int field, const Simulation_Params& p) const override {
         return field_size_for(field, p.fluid_count, p.boundary_count, p.bucket_count);
     }
 
@@ -444,41 +548,50 @@ class Engine final : public EngineBase {
         V4<R>* vec = nullptr;    // vector field
         R* scalar = nullptr;     // scalar field base
         uint32_t stride = 1, offset = 0, n = 0;
-        const void* raw = nullptr;  // uint32 field
+        const uint32_t* raw = nullptr;  // uint32 field
         size_t raw_words = 0;
-        bool boundary = false;
+        uint32_t raw_bias = 0;          // subtracted from every word of a host download
+        bool raw_pairs = false;         // (start, end) pairs: (0, 0) is an empty bucket
     };
 
-    FieldView view(int field) {
+    FieldView view(uint32_t s, int field) {
         FieldView v;
+        const uint32_t f = f_off_[s], b = b_off_[s], n = n_[s], nb = nb_[s];
+        const uint32_t B = s < buckets_.size() ? buckets_[s] : 0, base = s < buckets_.size() ? bucket_base_[s] : 0;
         switch (field) {
-            case PCISPH_FLUID_POSITIONS: v.vec = A_.pos_in; v.n = A_.n; break;
-            case PCISPH_FLUID_VELOCITIES: v.vec = A_.vel_in; v.n = A_.n; break;
-            case PCISPH_FLUID_NORMALS: v.vec = A_.nrm; v.n = A_.n; break;
-            case PCISPH_FLUID_PREDICTED_POSITIONS: v.vec = A_.xp; v.n = A_.n; break;
-            case PCISPH_FLUID_OTHER_FORCES: v.vec = A_.fo; v.n = A_.n; break;
-            case PCISPH_FLUID_PRESSURE_FORCES: v.vec = A_.fp; v.n = A_.n; break;
-            case PCISPH_FLUID_DENSITIES: v.scalar = (R*)A_.xr; v.stride = 4; v.offset = 3; v.n = A_.n; break;
-            case PCISPH_FLUID_PRESSURES: v.scalar = A_.p; v.n = A_.n; break;
-            case PCISPH_FLUID_DENSITY_VARIATIONS: v.scalar = A_.dv; v.n = A_.n; break;
-            case PCISPH_BOUNDARY_POSITIONS: v.vec = A_.bpos; v.n = A_.nb; v.boundary = true; break;
-            case PCISPH_BOUNDARY_PRESSURES: v.scalar = A_.bp; v.n = A_.nb; v.boundary = true; break;
-            case PCISPH_BOUNDARY_INIT_PRED_DENSITIES: v.scalar = A_.binit; v.n = A_.nb; v.boundary = true; break;
-            case PCISPH_FLUID_KEYS: v.raw = A_.keys[fluid_sorted_]; v.raw_words = A_.n; break;
-            case PCISPH_FLUID_SRC_LOCATIONS: v.raw = A_.vals[fluid_sorted_]; v.raw_words = A_.n; break;
-            case PCISPH_FLUID_CELL_OFFSETS: v.raw = A_.cells; v.raw_words = 2 * (size_t)A_.B; break;
-            case PCISPH_BOUNDARY_KEYS: v.raw = A_.bkeys[boundary_sorted_]; v.raw_words = A_.nb; break;
-            case PCISPH_BOUNDARY_SRC_LOCATIONS: v.raw = A_.bvals[boundary_sorted_]; v.raw_words = A_.nb; break;
-            case PCISPH_BOUNDARY_CELL_OFFSETS: v.raw = A_.bcells; v.raw_words = A_.nb ? 2 * (size_t)A_.B : 0; break;
-            case PCISPH_FLUID_NEIGHBOR_COUNTS: v.raw = A_.counts; v.raw_words = 4 * (size_t)A_.n; break;
+            case PCISPH_FLUID_POSITIONS: v.vec = A_.pos_in + f; v.n = n; break;
+            case PCISPH_FLUID_VELOCITIES: v.vec = A_.vel_in + f; v.n = n; break;
+            case PCISPH_FLUID_NORMALS: v.vec = A_.nrm + f; v.n = n; break;
+            case PCISPH_FLUID_PREDICTED_POSITIONS: v.vec = A_.xp + f; v.n = n; break;
+            case PCISPH_FLUID_OTHER_FORCES: v.vec = A_.fo + f; v.n = n; break;
+            case PCISPH_FLUID_PRESSURE_FORCES: v.vec = A_.fp + f; v.n = n; break;
+            case PCISPH_FLUID_DENSITIES: v.scalar = (R*)(A_.xr + f); v.stride = 4; v.offset = 3; v.n = n; break;
+            case PCISPH_FLUID_PRESSURES: v.scalar = A_.p + f; v.n = n; break;
+            case PCISPH_FLUID_DENSITY_VARIATIONS: v.scalar = A_.dv + f; v.n = n; break;
+            case PCISPH_BOUNDARY_POSITIONS: v.vec = A_.bpos + b; v.n = nb; break;
+            case PCISPH_BOUNDARY_PRESSURES: v.scalar = A_.bp + b; v.n = nb; break;
+            case PCISPH_BOUNDARY_INIT_PRED_DENSITIES: v.scalar = A_.binit + b; v.n = nb; break;
+            case PCISPH_FLUID_KEYS: v.raw = A_.keys[fluid_sorted_] + f; v.raw_words = n; v.raw_bias = base; break;
+            case PCISPH_FLUID_SRC_LOCATIONS: v.raw = A_.vals[fluid_sorted_] + f; v.raw_words = n; v.raw_bias = f; break;
+            case PCISPH_FLUID_CELL_OFFSETS:
+                v.raw = (const uint32_t*)(A_.cells + base); v.raw_words = 2 * (size_t)B; v.raw_bias = f;
+                v.raw_pairs = true;
+                break;
+            case PCISPH_BOUNDARY_KEYS: v.raw = A_.bkeys[boundary_sorted_] + b; v.raw_words = nb; v.raw_bias = base; break;
+            case PCISPH_BOUNDARY_SRC_LOCATIONS: v.raw = A_.bvals[boundary_sorted_] + b; v.raw_words = nb; v.raw_bias = b; break;
+            case PCISPH_BOUNDARY_CELL_OFFSETS:
+                v.raw = (const uint32_t*)(A_.bcells + base); v.raw_words = nb ? 2 * (size_t)B : 0; v.raw_bias = b;
+                v.raw_pairs = true;
+                break;
+            case PCISPH_FLUID_NEIGHBOR_COUNTS: v.raw = (const uint32_t*)(A_.counts + f); v.raw_words = 4 * (size_t)n; break;
             default: throw Error(PCISPH_ERR_INVALID_ARGUMENT, "unknown field " + std::to_string(field));
         }
         return v;
     }
 
-    void upload(int field, const void* src, size_t count, bool device) override {
+    void upload(uint32_t s, int field, const void* src, size_t count, bool device) override {
         HIP_CHECK(hipSetDevice(device_));
-        FieldView v = view(field);
+        FieldView v = view(s, field);
         if (v.raw) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "field is read-only (a product of the sort)");
         if (!count) return;
         const Launch L{stream_, 256};
@@ -493,16 +606,28 @@ class Engine final : public EngineBase {
         if (!device) HIP_CHECK(hipStreamSynchronize(stream_));  // staging reuse + host buffer lifetime
     }
 
-    void download(int field, void* dst, size_t count, bool device, bool f64) override {
+    void download(uint32_t s, int field, void* dst, size_t count, bool device, bool f64) override {
         HIP_CHECK(hipSetDevice(device_));
-        FieldView v = view(field);
+        FieldView v = view(s, field);
         if (!count) return;
         const Launch L{stream_, 256};
         if (v.raw) {
             if (f64) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "integer field has no FP64 view");
+            if (device && v.raw_bias)
+                throw Error(PCISPH_ERR_INVALID_ARGUMENT,
+                            "device download of a sort product is available for the first scene only");
             HIP_CHECK(hipMemcpyAsync(dst, v.raw, count * 4, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                      stream_));
             HIP_CHECK(hipStreamSynchronize(stream_));
+            if (v.raw_bias) {  // batch-wide indices and keys to the scene's own
+                uint32_t* w = (uint32_t*)dst;
+                if (v.raw_pairs) {
+                    for (size_t k = 0; k + 1 < count; k += 2)
+                        if (w[k] | w[k + 1]) w[k] -= v.raw_bias, w[k + 1] -= v.raw_bias;
+                } else {
+                    for (size_t k = 0; k < count; ++k) w[k] -= v.raw_bias;
+                }
+            }
             return;
         }
         void* to = device ? dst : (void*)staging_;
@@ -522,9 +647,11 @@ class Engine final : public EngineBase {
     void profile_enable(bool on) override { prof_.enable(on, stream_); }
     void profile_read(pcisph_stage_times* out) override { prof_.read(stream_, out); }
 
-    uint32_t nonfinite() override {
-        unsigned int v = 0;
-        HIP_CHECK(hipMemcpyAsync(&v, &A_.state->nonfinite, sizeof(v), hipMemcpyDeviceToHost, stream_));
+    std::vector<uint32_t> nonfinite() override {
+        std::vector<uint32_t> v(n_.size(), 0);
+        if (!v.empty())
+            HIP_CHECK(hipMemcpy2DAsync(v.data(), sizeof(uint32_t), &states_->nonfinite, sizeof(LoopState),
+                                       sizeof(uint32_t), v.size(), hipMemcpyDeviceToHost, stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
         return v;
     }
@@ -548,35 +675,90 @@ class Engine final : public EngineBase {
         }
     }
 
-    StepResult step(const Simulation_Params& p, float threshold, const pcisph_solver_options& o,
-                    bool sort_boundary, bool fast, const WarmArgs& w) override {
+    // The block table for workgroups of `block` threads: every scene's fluid
+    // particles in ranges of `block`, then every scene's boundary particles.
+    void prepare_blocks(uint32_t block) {
+        if (blocks_for_ == block) return;
+        std::vector<BlockRange> t;
+        for (uint32_t s = 0; s < n_.size(); ++s)
+            for (uint32_t b = f_off_[s]; b < f_off_[s + 1]; b += block)
+                t.push_back({s, b, std::min(b + block, f_off_[s + 1]), b == f_off_[s] ? kRangeFirstOfScene : 0u});
+        const uint32_t fluid_blocks = (uint32_t)t.size();
+        for (uint32_t s = 0; s < nb_.size(); ++s)
+            for (uint32_t b = b_off_[s]; b < b_off_[s + 1]; b += block)
+                t.push_back({s, b, std::min(b + block, b_off_[s + 1]), kRangeBoundary});
+        HIP_CHECK(hipStreamSynchronize(stream_));  // kernels enqueued ahead may still read the old table
+        dalloc(block_table_, std::max<size_t>(t.size(), 1), stream_);
+        if (!t.empty())
+            HIP_CHECK(hipMemcpyAsync(block_table_, t.data(), sizeof(BlockRange) * t.size(), hipMemcpyHostToDevice,
+                                     stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        fluid_blocks_ = fluid_blocks;
+        boundary_blocks_ = (uint32_t)t.size() - fluid_blocks;
+        blocks_for_ = block;
+    }
+
+    // The scene table of this step; uploaded when it differs from the last one.
+    void prepare_scenes(const std::vector<SceneInput>& scenes, const pcisph_solver_options& o) {
+        std::vector<SphParams<R>> t(scenes.size());
+        for (size_t s = 0; s < scenes.size(); ++s) {
+            const Simulation_Params& p = *scenes[s].params;
+            SphParams<R> P = make_sph_params<R>(p, scenes[s].threshold, o);
+            P.f_off = f_off_[s];
+            P.b_off = b_off_[s];
+            P.bucket_base = bucket_base_[s];
+            P.p_ref = (float)(P.delta * P.rho0 / R(100));
+            P.div_scale = (float)((double)p.particle_mass * 45.0 /
+                                  (3.14159265358979323846 * std::pow((double)P.h, 6)) * (double)P.dt * 100.0);
+            const float h = (float)P.h, radius = (1.f + list_skin_) * h;
+            P.list_r2 = list_skin_ > 0.f ? radius * radius : 0.f;
+            P.disp_limit = 0.45f * list_skin_ * h;
+            t[s] = P;
+        }
+        if (t.size() == scene_host_.size() &&
+            std::memcmp(t.data(), scene_host_.data(), sizeof(SphParams<R>) * t.size()) == 0)
+            return;
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        if (t.size() != scene_host_.size()) dalloc(scene_table_, std::max<size_t>(t.size(), 1), stream_);
+        HIP_CHECK(hipMemcpyAsync(scene_table_, t.data(), sizeof(SphParams<R>) * t.size(), hipMemcpyHostToDevice,
+                                 stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        scene_host_ = t;
+    }
+
+    void step(const std::vector<SceneInput>& scenes, const pcisph_solver_options& o, bool sort_boundary, bool fast,
+              const WarmArgs& w, std::vector<StepResult>& out, uint64_t* samples) override {
         HIP_CHECK(hipSetDevice(device_));
+        const uint32_t S = (uint32_t)scenes.size();
         const Launch L{stream_, o.block_size ? o.block_size : 256u};
         const MlpLaunch ML{stream_};
         const uint32_t digit = o.sort_digit_bits ? o.sort_digit_bits : 8u;
-        const SphParams<R> P = make_sph_params<R>(p, threshold, o);
-        const uint32_t key_bits = key_bits_for(p.bucket_count);
+        const uint32_t key_bits = key_bits_for(A_.B);
         const unsigned int tag = ++tag_ ? tag_ : ++tag_;
         const bool warm = w.params != nullptr;
         const bool features = warm || w.collect_stride;
+        prepare_blocks(L.block);
+        setup_lists(o.neighbor_skin);
+        prepare_scenes(scenes, o);
+        A_.batch = Batch<R>{scene_table_, block_table_, states_, S, fluid_blocks_, boundary_blocks_, A_.n, A_.nb};
         if (features) ensure_warm_buffers(warm);
-        setup_lists(o.neighbor_skin, P);
-        fill_u64(L, (unsigned long long*)A_.state, sizeof(LoopState) / 8, 0ull);
+        fill_u64(L, (unsigned long long*)states_, sizeof(LoopState) / 8 * (size_t)S, 0ull);
         if (sort_boundary && A_.nb) {
             prof_.timed(PCISPH_STAGE_BOUNDARY_SORT, stream_, -1, [&] {
                 fill_u32(L, (uint32_t*)A_.bcells, 2 * (size_t)A_.B, 0u);
-                DataLaunch<R>::hash(L, A_.bpos, A_.nb, P.cell, P.grid, A_.bkeys[0], A_.bvals[0]);
+                DataLaunch<R>::hash(L, A_.batch, true, A_.bpos, A_.bkeys[0], A_.bvals[0]);
                 boundary_sorted_ = radix_sort(L, A_.bkeys, A_.bvals, A_.nb, key_bits, digit, sort_scratch_);
                 if (boundary_sorted_ < 0) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "too many boundary particles");
                 DataLaunch<R>::reorder(L, A_.nb, A_.bkeys[boundary_sorted_], A_.bvals[boundary_sorted_],
                                        A_.bpos, nullptr, A_.bpos_tmp, nullptr, A_.bcells);
-                std::swap(A_.bpos, A_.bpos_tmp);
-                if (o.launch_boundary_init) launch(fast).boundary_init(L, P, A_);
+                HIP_CHECK(hipMemcpyAsync(A_.bpos, A_.bpos_tmp, sizeof(V4<R>) * A_.nb, hipMemcpyDeviceToDevice,
+                                         stream_));
+                if (o.launch_boundary_init) launch(fast).boundary_init(L, A_);
             });
         }
         prof_.timed(PCISPH_STAGE_HASH, stream_, -1, [&] {
             fill_u32(L, (uint32_t*)A_.cells, 2 * (size_t)A_.B, 0u);
-            DataLaunch<R>::hash(L, A_.pos_in, A_.n, P.cell, P.grid, A_.keys[0], A_.vals[0]);
+            DataLaunch<R>::hash(L, A_.batch, false, A_.pos_in, A_.keys[0], A_.vals[0]);
         });
         prof_.timed(PCISPH_STAGE_SORT, stream_, -1, [&] {
             fluid_sorted_ = radix_sort(L, A_.keys, A_.vals, A_.n, key_bits, digit, sort_scratch_);
@@ -587,19 +769,17 @@ class Engine final : public EngineBase {
                                    A_.vel_in, A_.pos, A_.vel, A_.cells);
         });
         const LaunchTable ops = launch(fast);
-        prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, P, A_); });
-        prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, P, A_); });
-        const float p_ref = (float)(P.delta * P.rho0 / R(100));
-        FeatureArgs F{features ? feat_ : nullptr, A_.counts, p_ref,
-                      (float)((double)p.particle_mass * 45.0 / (3.14159265358979323846 * std::pow((double)P.h, 6)) *
-                              (double)P.dt * 100.0)};
-        prof_.timed(PCISPH_STAGE_FORCE_INIT, stream_, -1, [&] { ops.force_init(L, P, A_, F); });
+        prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
+        prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, A_); });
+        FeatureArgs F{features ? feat_ : nullptr, A_.counts};
+        prof_.timed(PCISPH_STAGE_FORCE_INIT, stream_, -1, [&] { ops.force_init(L, A_, F); });
         if (warm) {
             // network start: p from the features, its pressure force, and a prediction
             // with it; the plain prediction is kept for the dual first iteration
             prof_.timed(PCISPH_STAGE_WARM_START, stream_, -1, [&] {
-                mlp_warm_start<R, V4<R>>(ML, w.params, feat_, A_.n, p_ref, A_.xr, A_.p, A_.xf);
-                ops.pressure_force(L, P, A_, 0, dev_host_, tag, 1, D_.xp0);
+                mlp_warm_start<R>(ML, w.params, feat_, A_.n, A_.p);
+                DataLaunch<R>::warm_apply(L, A_);
+                ops.pressure_force(L, A_, 0, dev_host_, dev_results_, tag, 1, D_.xp0);
             });
         }
         HIP_CHECK(hipGetLastError());
@@ -610,11 +790,11 @@ class Engine final : public EngineBase {
         uint32_t enq = 0;
         auto enqueue = [&]() {
             prof_.timed(PCISPH_STAGE_PRESSURE, stream_, (int)enq, [&] {
-                if (warm && enq == 0) ops.pressure_dual(L, P, A_, D_);
-                else ops.pressure(L, P, A_, enq);
+                if (warm && enq == 0) ops.pressure_dual(L, A_, D_);
+                else ops.pressure(L, A_, enq);
             });
             prof_.timed(PCISPH_STAGE_PRESSURE_FORCE, stream_, (int)enq, [&] {
-                ops.pressure_force(L, P, A_, enq, dev_host_, tag, 0, nullptr);
+                ops.pressure_force(L, A_, enq, dev_host_, dev_results_, tag, 0, nullptr);
             });
             ++enq;
         };
@@ -623,41 +803,58 @@ class Engine final : public EngineBase {
         uint32_t k = first;
         for (;;) {
             wait_for_iteration(k, tag);
-            if (iteration_is_last<R>((const unsigned long long*)host_->max_dv, k, minit, maxit, P.rho0,
-                                     P.threshold))
-                break;
+            if (host_->running[k] == 0) break;
+            if (k + 1 >= maxit) throw Error(PCISPH_ERR_INTERNAL, "scenes still running after the last iteration");
             ++k;
             if (enq < maxit) enqueue();
         }
         HIP_CHECK(hipGetLastError());
-        StepResult r;
+        uint64_t collected = 0;
         if (w.collect_stride) {
             const uint32_t phase = w.collect_phase % w.collect_stride;
-            mlp_collect<R>(ML, feat_, A_.p, A_.n, w.collect_stride, phase, p_ref, w.store_x, w.store_t,
-                           w.store_base, w.store_cap);
-            r.samples = A_.n > phase ? (A_.n - phase + w.collect_stride - 1) / w.collect_stride : 0;
+            uint64_t base = w.store_base;
+            for (uint32_t s = 0; s < S; ++s) {
+                const uint32_t n = n_[s];
+                if (!n || base >= w.store_cap) continue;
+                const float p_ref = scene_host_[s].p_ref;
+                mlp_collect<R>(ML, feat_ + (size_t)f_off_[s] * 8, A_.p + f_off_[s], n, w.collect_stride, phase, p_ref,
+                               w.store_x, w.store_t, base, w.store_cap);
+                const uint64_t c = n > phase ? (n - phase + w.collect_stride - 1) / w.collect_stride : 0;
+                base += c;
+                collected += c;
+            }
+            HIP_CHECK(hipGetLastError());
         }
+        if (samples) *samples = collected;
         prof_.end_step((int)k);
         prof_.maybe_resolve(stream_);
-        r.iterations = k + 1;
-        const R mdv = Bits<R>::to(host_->max_dv[k]);
-        r.max_dv = (double)mdv;
-        r.max_expansion = (double)Bits<R>::to(host_->max_exp[k]);
-        r.converged = (k + 1 >= minit) && (mdv / P.rho0 < P.threshold);
-        r.fluid_candidates = host_->candidates[0];
-        r.boundary_candidates = host_->candidates[1];
-        r.warm = warm;
-        r.warm_rejected = warm && host_->warm_rejected;
-        r.list_passes = host_->list_passes;
-        return r;
+        out.assign(S, StepResult{});
+        for (uint32_t s = 0; s < S; ++s) {
+            if (!n_[s]) continue;
+            const volatile SceneResult& q = results_[s];
+            StepResult& r = out[s];
+            r.iterations = q.iterations;
+            const R mdv = Bits<R>::to(q.max_dv);
+            const R rho0 = scene_host_[s].rho0;
+            r.max_dv = (double)mdv;
+            r.max_expansion = (double)Bits<R>::to(q.max_exp);
+            r.converged = (r.iterations >= minit) && (mdv / rho0 < scene_host_[s].threshold);
+            r.fluid_candidates = q.candidates[0];
+            r.boundary_candidates = q.candidates[1];
+            r.warm = warm;
+            r.warm_rejected = warm && q.warm_rejected;
+            r.list_passes = q.list_passes;
+        }
     }
 
     // Neighbour-list storage (capacities: fluid 128, boundary 96 per fluid particle,
-    // fluid 96 per boundary particle) and this step's radius and validity limit.
-    void setup_lists(float skin, const SphParams<R>& P) {
+    // fluid 96 per boundary particle); each scene's list radius and validity limit
+    // follow from its kernel radius (scene table).
+    void setup_lists(float skin) {
         NeighborLists& NL = A_.NL;
+        list_skin_ = skin > 0.f ? skin : 0.f;
         if (!(skin > 0.f)) {
-            NL.r2 = 0.f;
+            NL.on = 0;
             return;
         }
         constexpr uint32_t kCapF = 128, kCapB = 96, kCapBF = 96;
@@ -674,9 +871,7 @@ class Engine final : public EngineBase {
             list_n_ = A_.n;
             list_nb_ = A_.nb;
         }
-        const float radius = (1.f + skin) * (float)P.h;
-        NL.r2 = radius * radius;
-        NL.disp_limit = 0.45f * skin * (float)P.h;
+        NL.on = 1;
     }
 
     void ensure_warm_buffers(bool dual) {
@@ -699,14 +894,14 @@ class Engine final : public EngineBase {
 
    private:
     struct LaunchTable {
-        void (*density)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&);
-        void (*normal)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&);
-        void (*force_init)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, const FeatureArgs&);
-        void (*pressure)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, uint32_t);
-        void (*pressure_force)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, uint32_t, HostMirror*,
+        void (*density)(const Launch&, const DeviceArrays<R>&);
+        void (*normal)(const Launch&, const DeviceArrays<R>&);
+        void (*force_init)(const Launch&, const DeviceArrays<R>&, const FeatureArgs&);
+        void (*pressure)(const Launch&, const DeviceArrays<R>&, uint32_t);
+        void (*pressure_force)(const Launch&, const DeviceArrays<R>&, uint32_t, HostMirror*, SceneResult*,
                                unsigned int, int, V4<R>*);
-        void (*boundary_init)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&);
-        void (*pressure_dual)(const Launch&, const SphParams<R>&, const DeviceArrays<R>&, const DualArrays<R>&);
+        void (*boundary_init)(const Launch&, const DeviceArrays<R>&);
+        void (*pressure_dual)(const Launch&, const DeviceArrays<R>&, const DualArrays<R>&);
     };
     static LaunchTable launch(bool fast);
 
@@ -714,6 +909,16 @@ class Engine final : public EngineBase {
     hipStream_t stream_;
     DeviceArrays<R> A_;
     DualArrays<R> D_{};
+    // scene layout: counts, first particle, bucket counts and first bucket per scene
+    std::vector<uint32_t> n_{0}, nb_{0}, f_off_{0, 0}, b_off_{0, 0}, buckets_, bucket_base_{0};
+    std::vector<SphParams<R>> scene_host_;
+    SphParams<R>* scene_table_ = nullptr;
+    BlockRange* block_table_ = nullptr;
+    uint32_t blocks_for_ = 0, fluid_blocks_ = 0, boundary_blocks_ = 0;
+    LoopState* states_ = nullptr;
+    uint32_t states_n_ = 0;
+    volatile SceneResult* results_ = nullptr;
+    SceneResult* dev_results_ = nullptr;
     int fluid_sorted_ = 0, boundary_sorted_ = 0;
     uint32_t* sort_scratch_ = nullptr;
     size_t sort_scratch_words_ = 0;
@@ -721,6 +926,11 @@ class Engine final : public EngineBase {
     size_t staging_words_ = 0;
     float* feat_ = nullptr;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
+    float list_skin_ = 0.f;
+    float* shift_host_ = nullptr;  // pinned staging of pcisph_translate_boundaries
+    float* shift_dev_ = nullptr;
+    size_t shift_cap_ = 0;
+    hipEvent_t shift_event_ = nullptr;
     V4<R>* snap_pos_ = nullptr;
     V4<R>* snap_vel_ = nullptr;
     V4<R>* snap_b_ = nullptr;
@@ -759,13 +969,20 @@ using namespace pcisph;
 enum { kPrecisionFast = 0, kPrecisionIeee = 1, kPrecisionFp64 = 2 };
 
 struct pcisph_solver {
+    // One scene of the batch: what a sim::Fluid holds (Fluid.h:13-102) -- its
+    // parameters, whether a setter changed them since the last step (Fluid.cpp:109-112),
+    // and its loop's density-variation threshold.
+    struct Scene {
+        Simulation_Params params{};
+        bool params_changed = true;
+        bool boundary_updated = true;
+        float threshold = 0.01f;
+    };
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    Simulation_Params params{};
-    bool params_changed = true;
-    bool boundary_updated = true;
-    float threshold = 0.01f;
+    std::vector<Scene> scenes = std::vector<Scene>(1);
+    uint32_t current = 0;  // the scene that setters, uploads and downloads address
     pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 0};
     int precision = kPrecisionFast;
     std::unique_ptr<EngineBase> eng;
@@ -786,6 +1003,24 @@ struct pcisph_solver {
     ~pcisph_solver() {
         for (float* q : {mlp, mlp_m, mlp_v, mlp_grad, mlp_loss, samples_x, samples_t})
             if (q) (void)hipFree(q);
+    }
+
+    Scene& cur() { return scenes[current]; }
+    const Scene& cur() const { return scenes[current]; }
+
+    void push_counts() {
+        std::vector<uint32_t> n(scenes.size()), nb(scenes.size());
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            n[i] = scenes[i].params.fluid_count;
+            nb[i] = scenes[i].params.boundary_count;
+        }
+        eng->set_counts(n, nb);
+    }
+
+    uint64_t total_fluid() const {
+        uint64_t t = 0;
+        for (const Scene& sc : scenes) t += sc.params.fluid_count;
+        return t;
     }
 
     void ensure_mlp() {
@@ -824,8 +1059,8 @@ static int guarded(pcisph_solver* s, Fn&& fn) {
 static void make_engine(pcisph_solver* s) {
     if (s->precision == kPrecisionFp64) s->eng.reset(new Engine<double>(s->device, s->stream));
     else s->eng.reset(new Engine<float>(s->device, s->stream));
-    s->eng->resize_public(s->params.fluid_count, s->params.boundary_count);
-    s->params_changed = true;
+    s->push_counts();
+    for (auto& sc : s->scenes) sc.params_changed = true;
 }
 
 extern "C" {
@@ -882,9 +1117,9 @@ const char* pcisph_last_error(const pcisph_solver* s) { return s ? s->err.c_str(
 #define PCISPH_SETTER(name, type, member, resize)                                      \
     int pcisph_set_##name(pcisph_solver* s, type v) {                                  \
         return guarded(s, [&] {                                                        \
-            s->params.member = v;                                                      \
-            s->params_changed = true;                                                  \
-            if (resize) s->eng->resize_public(s->params.fluid_count, s->params.boundary_count); \
+            s->cur().params.member = v;                                                \
+            s->cur().params_changed = true;                                            \
+            if (resize) s->push_counts();                                              \
         });                                                                            \
     }
 PCISPH_SETTER(boundary_count, uint32_t, boundary_count, true)
@@ -898,12 +1133,12 @@ PCISPH_SETTER(surface_tension, float, surface_tension_coefficient, false)
 #undef PCISPH_SETTER
 
 int pcisph_set_density_variation_threshold(pcisph_solver* s, float threshold) {
-    return guarded(s, [&] { s->threshold = threshold; });
+    return guarded(s, [&] { s->cur().threshold = threshold; });
 }
 
 int pcisph_get_params(const pcisph_solver* s, Simulation_Params* out) {
     if (!s || !out) return PCISPH_ERR_INVALID_ARGUMENT;
-    *out = s->params;
+    *out = s->cur().params;
     return PCISPH_OK;
 }
 
@@ -945,19 +1180,20 @@ int pcisph_check_buffers_consistent(const pcisph_solver* s) {
 
 int pcisph_field_size(const pcisph_solver* s, int field, size_t* out) {
     if (!s || !out) return PCISPH_ERR_INVALID_ARGUMENT;
-    const size_t n = s->eng->field_size(field, s->params);
+    const size_t n = s->eng->field_size(s->current, field, s->cur().params);
     if (n == (size_t)-1) return PCISPH_ERR_INVALID_ARGUMENT;
     *out = n;
     return PCISPH_OK;
 }
 
 static void check_count(pcisph_solver* s, int field, size_t count) {
-    const size_t n = s->eng->field_size(field, s->params);
+    const Simulation_Params& p = s->cur().params;
+    const size_t n = s->eng->field_size(s->current, field, p);
     if (n == (size_t)-1) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "unknown field " + std::to_string(field));
     if (count != n)
         throw Error(PCISPH_ERR_SIZE, "Inconsistent size (" + std::to_string(count) + ") of field " +
-                                         std::to_string(field) + " for " + std::to_string(s->params.fluid_count) +
-                                         " fluid / " + std::to_string(s->params.boundary_count) +
+                                         std::to_string(field) + " for " + std::to_string(p.fluid_count) +
+                                         " fluid / " + std::to_string(p.boundary_count) +
                                          " boundary particles (expected " + std::to_string(n) + ")");
 }
 
@@ -965,8 +1201,8 @@ static int upload_impl(pcisph_solver* s, int field, const void* p, size_t count,
     return guarded(s, [&] {
         if (!p && count) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "null pointer");
         check_count(s, field, count);
-        s->eng->upload(field, p, count, device);
-        if (field == PCISPH_BOUNDARY_POSITIONS) s->boundary_updated = true;
+        s->eng->upload(s->current, field, p, count, device);
+        if (field == PCISPH_BOUNDARY_POSITIONS) s->cur().boundary_updated = true;
     });
 }
 
@@ -974,7 +1210,7 @@ static int download_impl(pcisph_solver* s, int field, void* p, size_t count, boo
     return guarded(s, [&] {
         if (!p && count) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "null pointer");
         check_count(s, field, count);
-        s->eng->download(field, p, count, device, f64);
+        s->eng->download(s->current, field, p, count, device, f64);
     });
 }
 
@@ -994,60 +1230,161 @@ int pcisph_download_f64(pcisph_solver* s, int field, double* host, size_t count)
     return download_impl(s, field, host, count, false, true);
 }
 
+// update_deduced_attributes for every scene whose parameters changed
+// (Fluid.cpp:109-112, 377-466): derived parameters, bucket tables, and the scene's
+// velocities zeroed (Fluid.cpp:462-463).
 static void apply_param_changes(pcisph_solver* s) {
-    if (!s->params_changed) return;
-    derive_params(s->params);
-    s->eng->realloc_internal(s->params);
-    s->eng->zero_velocities();  // Fluid.cpp:462-463
-    s->boundary_updated = true;
-    s->params_changed = false;
+    bool any = false;
+    for (auto& sc : s->scenes) any = any || sc.params_changed;
+    if (!any) return;
+    std::vector<pcisph::SceneInput> in;
+    for (auto& sc : s->scenes) {
+        if (sc.params_changed) derive_params(sc.params);
+        in.push_back({&sc.params, sc.threshold});
+    }
+    s->eng->realloc_internal(in);
+    for (uint32_t i = 0; i < s->scenes.size(); ++i) {
+        auto& sc = s->scenes[i];
+        if (!sc.params_changed) continue;
+        s->eng->zero_velocities(i);
+        sc.boundary_updated = true;
+        sc.params_changed = false;
+    }
 }
 
-static void step_impl(pcisph_solver* s, pcisph_step_stats* stats) {
-    apply_param_changes(s);
+static pcisph_step_stats to_stats(const pcisph_solver* s, const StepResult& r, const Simulation_Params& p) {
     pcisph_step_stats st{};
-    if (s->params.fluid_count) {
-        WarmArgs w;
-        if (s->warm_start && s->mlp) w.params = s->mlp;
-        if (s->collect_stride && s->samples_x && s->samples_count < s->samples_cap) {
-            w.collect_stride = s->collect_stride;
-            w.collect_phase = s->collect_phase++;
-            w.store_x = s->samples_x;
-            w.store_t = s->samples_t;
-            w.store_base = s->samples_count;
-            w.store_cap = s->samples_cap;
-        }
-        const StepResult r = s->eng->step(s->params, s->threshold, s->opt,
-                                          s->boundary_updated || s->opt.resort_boundary,
-                                          s->precision == kPrecisionFast, w);
-        s->samples_count = std::min(s->samples_cap, s->samples_count + r.samples);
-        st.warm_start = (r.warm ? 1u : 0u) | (r.warm_rejected ? 2u : 0u);
-        st.max_density_expansion = (float)r.max_expansion;
-        st.list_passes = r.list_passes;
-        s->boundary_updated = false;
-        st.iterations = r.iterations;
-        st.converged = r.converged;
-        st.max_density_variation = (float)r.max_dv;
-        st.max_density_error = (float)(r.max_dv / (double)s->params.rest_density);
-        if (s->precision != kPrecisionFp64)
-            st.max_density_error = (float)r.max_dv / s->params.rest_density;
-        st.fluid_candidates = r.fluid_candidates;
-        st.boundary_candidates = r.boundary_candidates;
+    if (!p.fluid_count) return st;
+    st.warm_start = (r.warm ? 1u : 0u) | (r.warm_rejected ? 2u : 0u);
+    st.max_density_expansion = (float)r.max_expansion;
+    st.list_passes = r.list_passes;
+    st.iterations = r.iterations;
+    st.converged = r.converged;
+    st.max_density_variation = (float)r.max_dv;
+    st.max_density_error = (float)(r.max_dv / (double)p.rest_density);
+    if (s->precision != kPrecisionFp64) st.max_density_error = (float)r.max_dv / p.rest_density;
+    st.fluid_candidates = r.fluid_candidates;
+    st.boundary_candidates = r.boundary_candidates;
+    return st;
+}
+
+// The batch's statistics as one step's: the largest iteration count, error and
+// expansion, converged if every scene converged, candidates summed.
+static pcisph_step_stats combine(const std::vector<pcisph_step_stats>& v) {
+    pcisph_step_stats a{};
+    if (v.empty()) return a;
+    a.converged = 1;
+    bool any = false;
+    for (const auto& st : v) {
+        if (!st.iterations) continue;
+        any = true;
+        a.iterations = std::max(a.iterations, st.iterations);
+        a.converged = a.converged && st.converged;
+        a.max_density_variation = std::max(a.max_density_variation, st.max_density_variation);
+        a.max_density_error = std::max(a.max_density_error, st.max_density_error);
+        a.max_density_expansion = std::max(a.max_density_expansion, st.max_density_expansion);
+        a.nonfinite |= st.nonfinite;
+        a.warm_start |= st.warm_start;
+        a.fluid_candidates += st.fluid_candidates;
+        a.boundary_candidates += st.boundary_candidates;
+        a.list_passes = std::max(a.list_passes, st.list_passes);
     }
-    if (stats) *stats = st;
+    if (!any) a.converged = 0;
+    return a;
+}
+
+// One step of every scene; per-scene statistics into `out` (resized to the scene count).
+static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
+    apply_param_changes(s);
+    if (out) out->assign(s->scenes.size(), pcisph_step_stats{});
+    if (!s->total_fluid()) return;
+    WarmArgs w;
+    if (s->warm_start && s->mlp) w.params = s->mlp;
+    if (s->collect_stride && s->samples_x && s->samples_count < s->samples_cap) {
+        w.collect_stride = s->collect_stride;
+        w.collect_phase = s->collect_phase++;
+        w.store_x = s->samples_x;
+        w.store_t = s->samples_t;
+        w.store_base = s->samples_count;
+        w.store_cap = s->samples_cap;
+    }
+    std::vector<pcisph::SceneInput> in;
+    bool sort_boundary = s->opt.resort_boundary != 0;
+    for (auto& sc : s->scenes) {
+        in.push_back({&sc.params, sc.threshold});
+        sort_boundary = sort_boundary || sc.boundary_updated;
+    }
+    std::vector<StepResult> res;
+    uint64_t samples = 0;
+    s->eng->step(in, s->opt, sort_boundary, s->precision == kPrecisionFast, w, res, &samples);
+    s->samples_count = std::min(s->samples_cap, s->samples_count + samples);
+    for (auto& sc : s->scenes) sc.boundary_updated = false;
+    if (out)
+        for (size_t i = 0; i < res.size(); ++i) (*out)[i] = to_stats(s, res[i], s->scenes[i].params);
+}
+
+static void mark_nonfinite(pcisph_solver* s, std::vector<pcisph_step_stats>& v) {
+    if (!s->total_fluid()) return;
+    const std::vector<uint32_t> bad = s->eng->nonfinite();
+    for (size_t i = 0; i < v.size() && i < bad.size(); ++i) v[i].nonfinite = bad[i];
 }
 
 int pcisph_step(pcisph_solver* s, pcisph_step_stats* stats) {
     return guarded(s, [&] {
-        step_impl(s, stats);
-        if (stats && s->params.fluid_count) stats->nonfinite = s->eng->nonfinite();
+        std::vector<pcisph_step_stats> v;
+        step_impl(s, stats ? &v : nullptr);
+        if (!stats) return;
+        mark_nonfinite(s, v);
+        *stats = combine(v);
     });
 }
 
 int pcisph_run(pcisph_solver* s, uint32_t n_steps, pcisph_step_stats* stats) {
     return guarded(s, [&] {
-        for (uint32_t i = 0; i < n_steps; i++) step_impl(s, stats ? stats + i : nullptr);
-        if (stats && n_steps && s->params.fluid_count) stats[n_steps - 1].nonfinite = s->eng->nonfinite();
+        std::vector<pcisph_step_stats> v;
+        for (uint32_t i = 0; i < n_steps; i++) {
+            step_impl(s, stats ? &v : nullptr);
+            if (!stats) continue;
+            if (i + 1 == n_steps) mark_nonfinite(s, v);
+            stats[i] = combine(v);
+        }
+    });
+}
+
+int pcisph_step_scenes(pcisph_solver* s, pcisph_step_stats* stats, size_t count) {
+    return guarded(s, [&] {
+        if (stats && count != s->scenes.size())
+            throw Error(PCISPH_ERR_SIZE, "expected statistics for " + std::to_string(s->scenes.size()) + " scenes");
+        std::vector<pcisph_step_stats> v;
+        step_impl(s, stats ? &v : nullptr);
+        if (!stats) return;
+        mark_nonfinite(s, v);
+        std::copy(v.begin(), v.end(), stats);
+    });
+}
+
+int pcisph_set_scene_count(pcisph_solver* s, uint32_t count) {
+    return guarded(s, [&] {
+        if (count == 0 || count > (1u << 20)) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "scene count must be in 1..2^20");
+        if (count == s->scenes.size()) return;
+        s->scenes.resize(count);
+        if (s->current >= count) s->current = 0;
+        for (auto& sc : s->scenes) sc.params_changed = true;  // bucket layout of the batch
+        s->push_counts();
+    });
+}
+
+int pcisph_get_scene_count(const pcisph_solver* s, uint32_t* count) {
+    if (!s || !count) return PCISPH_ERR_INVALID_ARGUMENT;
+    *count = (uint32_t)s->scenes.size();
+    return PCISPH_OK;
+}
+
+int pcisph_select_scene(pcisph_solver* s, uint32_t index) {
+    return guarded(s, [&] {
+        if (index >= s->scenes.size())
+            throw Error(PCISPH_ERR_INVALID_ARGUMENT, "scene " + std::to_string(index) + " out of range");
+        s->current = index;
     });
 }
 
@@ -1195,8 +1532,19 @@ int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float lr,
 
 int pcisph_translate_boundary(pcisph_solver* s, float dx, float dy, float dz) {
     return guarded(s, [&] {
-        s->eng->translate_boundary(dx, dy, dz);
-        s->boundary_updated = true;
+        s->eng->translate_boundary(s->current, dx, dy, dz);
+        s->cur().boundary_updated = true;
+    });
+}
+
+int pcisph_translate_boundaries(pcisph_solver* s, const float* shifts, size_t count) {
+    return guarded(s, [&] {
+        if (!shifts || count != 3 * s->scenes.size())
+            throw Error(PCISPH_ERR_SIZE, "expected 3 shifts per scene (" + std::to_string(3 * s->scenes.size()) + ")");
+        s->eng->translate_boundaries(shifts, s->opt.block_size ? s->opt.block_size : 256u);
+        for (uint32_t i = 0; i < s->scenes.size(); ++i)
+            if (shifts[3 * i] != 0.f || shifts[3 * i + 1] != 0.f || shifts[3 * i + 2] != 0.f)
+                s->scenes[i].boundary_updated = true;
     });
 }
 
@@ -1214,11 +1562,15 @@ int pcisph_set_warm_start(pcisph_solver* s, int enable) {
 // solver options.  The state after the call is the state before it.
 int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen) {
     return guarded(s, [&] {
-        if (!s->params.fluid_count) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "no particles to tune on");
+        if (!s->total_fluid()) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "no particles to tune on");
         if (!trial_steps) trial_steps = 3;
         apply_param_changes(s);
         // the boundary sort a pending update would do is part of every trial's first step
-        const bool boundary_pending = s->boundary_updated;
+        std::vector<bool> boundary_pending;
+        for (auto& sc : s->scenes) boundary_pending.push_back(sc.boundary_updated);
+        auto reset_boundary = [&] {
+            for (size_t i = 0; i < s->scenes.size(); ++i) s->scenes[i].boundary_updated = boundary_pending[i];
+        };
         s->eng->snapshot();
         const pcisph_solver_options base = s->opt;
         const uint32_t collect = s->collect_stride;
@@ -1230,10 +1582,10 @@ int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_option
             s->opt.block_size = block;
             s->opt.sort_digit_bits = digit;
             s->eng->restore();
-            s->boundary_updated = boundary_pending;
+            reset_boundary();
             step_impl(s, nullptr);  // warm-up: caches, code objects
             s->eng->restore();
-            s->boundary_updated = boundary_pending;
+            reset_boundary();
             HIP_CHECK(hipStreamSynchronize(s->stream));
             const auto t0 = std::chrono::steady_clock::now();
             for (uint32_t i = 0; i < trial_steps; ++i) step_impl(s, nullptr);
@@ -1250,7 +1602,7 @@ int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_option
         s->opt = best_opt;
         s->collect_stride = collect;
         s->eng->restore();
-        s->boundary_updated = boundary_pending;
+        reset_boundary();
         if (chosen) *chosen = best_opt;
     });
 }

@@ -56,7 +56,29 @@ __global__ void k_translate(V4<R>* __restrict__ p, uint32_t n, R dx, R dy, R dz)
     p[i] = v;
 }
 
+// Every boundary range moved by its scene's shift[3 * scene ..].
+template <typename R>
+__global__ void k_translate_scenes(const BlockRange* __restrict__ blocks, const float* __restrict__ shift,
+                                   V4<R>* __restrict__ p) {
+    const BlockRange rg = blocks[blockIdx.x];
+    const uint32_t i = rg.begin + threadIdx.x;
+    if (i >= rg.end) return;
+    const R dx = (R)shift[3 * rg.scene], dy = (R)shift[3 * rg.scene + 1], dz = (R)shift[3 * rg.scene + 2];
+    V4<R> v = p[i];
+    v.x = v.x + dx;
+    v.y = v.y + dy;
+    v.z = v.z + dz;
+    p[i] = v;
+}
+
 }  // namespace
+
+template <typename R>
+void DataLaunch<R>::translate_scenes(const Launch& L, const Batch<R>& B, const float* shift, V4<R>* bpos) {
+    if (!B.boundary_blocks) return;
+    hipLaunchKernelGGL((k_translate_scenes<R>), dim3(B.boundary_blocks), dim3(L.block), 0, L.stream,
+                       B.blocks + B.fluid_blocks, shift, bpos);
+}
 
 template <typename R>
 void DataLaunch<R>::translate(const Launch& L, V4<R>* p, uint32_t n, float dx, float dy, float dz) {
@@ -65,11 +87,19 @@ void DataLaunch<R>::translate(const Launch& L, V4<R>* p, uint32_t n, float dx, f
 }
 
 template <typename R>
-void DataLaunch<R>::hash(const Launch& L, const V4<R>* pos, uint32_t n, R cell, const HashGrid& g,
-                         uint32_t* keys, uint32_t* vals) {
-    if (!n) return;
-    hipLaunchKernelGGL((k_hash<R>), dim3(grid_for(n, L.block)), dim3(L.block), 0, L.stream, pos, n, cell, g,
-                       keys, vals);
+void DataLaunch<R>::hash(const Launch& L, const Batch<R>& B, bool boundary, const V4<R>* pos, uint32_t* keys,
+                         uint32_t* vals) {
+    const uint32_t grid = boundary ? B.boundary_blocks : B.fluid_blocks;
+    if (!grid) return;
+    hipLaunchKernelGGL((k_hash<R>), dim3(grid), dim3(L.block), 0, L.stream, B.scenes,
+                       B.blocks + (boundary ? B.fluid_blocks : 0u), pos, keys, vals);
+}
+
+template <typename R>
+void DataLaunch<R>::warm_apply(const Launch& L, const DeviceArrays<R>& A) {
+    if (!A.batch.fluid_blocks) return;
+    hipLaunchKernelGGL((k_warm_apply<R>), dim3(A.batch.fluid_blocks), dim3(L.block), 0, L.stream, A.batch, A.xr,
+                       A.p, A.xf);
 }
 
 template <typename R>

@@ -19,6 +19,17 @@
 //   bpos  float4 (x, y, z, 0), bxf float4 (x, y, z, p_b/rho0^2) for the boundary
 // Every kernel is one work-item per particle in the sorted order, so the 64 lanes of
 // a wavefront walk neighbouring buckets together and share their loads.
+//
+// Batches.  A launch covers every scene of a batch.  The scenes' particles are
+// concatenated (scene s owns fluid [f_off, f_off + n) and boundary
+// [b_off, b_off + nb)), their bucket tables too (scene s owns buckets
+// [bucket_base, bucket_base + B)), and a block table gives each workgroup one scene
+// and one contiguous range of its fluid or boundary particles.  The scene's
+// constants are therefore uniform across the workgroup and read with scalar loads,
+// and a batch of one scene is the single-scene solver.  Sorting the concatenated
+// keys bucket_base + hash keeps every scene contiguous and in the order its own
+// sort would give (the sort is stable and the scenes' key ranges are disjoint and
+// increasing).
 #pragma once
 
 #include "device_common.h"
@@ -27,9 +38,9 @@ namespace pcisph {
 
 constexpr int kMaxIterationsCap = 256;
 
-// Step-constant scalars, each product formed once on the host in the same order as
-// the reference kernels write it (e.g. `density *= particle_mass * poly6_normalization`,
-// PCISPH/data/kernels/pcisph.cl:90).
+// Step-constant scalars of one scene, each product formed once on the host in the
+// same order as the reference kernels write it (e.g. `density *= particle_mass *
+// poly6_normalization`, PCISPH/data/kernels/pcisph.cl:90).
 template <typename R>
 struct SphParams {
     R h, h2, cell, rho0, m, dt, delta, threshold;
@@ -43,8 +54,22 @@ struct SphParams {
     R gravity_force;   // m * g                         (pcisph.cl:167)
     R rho0_sq;         // rho0 * rho0                   (pcisph.cl:282-283)
     HashGrid grid;
-    uint32_t n, nb;
+    uint32_t n, nb;                  // this scene's fluid / boundary counts
+    uint32_t f_off, b_off;           // its first fluid / boundary particle in the batch
+    uint32_t bucket_base;            // its first bucket in the batch's cell tables
     uint32_t min_iterations, max_iterations;
+    float p_ref;                     // delta * rho0 / 100: warm-start pressure unit
+    float div_scale;                 // m * 45 / (pi h^6) * dt * 100: warm-start divergence unit
+    float list_r2;                   // neighbour-list radius squared, ((1 + skin) h)^2
+    float disp_limit;                // largest max |xp - x| with which a pressure pass may use the lists
+};
+
+// One workgroup's share of a launch: particles [begin, end) of one scene.
+enum : uint32_t { kRangeBoundary = 1u, kRangeFirstOfScene = 2u };
+struct BlockRange {
+    uint32_t scene;
+    uint32_t begin, end;  // indices into the batch's fluid (or boundary) arrays
+    uint32_t flags;       // kRangeBoundary, kRangeFirstOfScene (first fluid range of its scene)
 };
 
 // Per-step device record.  max_dv[k] holds the bits of the non-negative maximum
@@ -77,15 +102,24 @@ struct DualArrays {
     V4<R>* bxf0;
 };
 
-// Host-visible copy of the loop's per-iteration results (pinned, mapped memory),
-// written by the first thread of each pressure-force launch.
+// Host-visible record of the correction loop (pinned, mapped memory), written by the
+// first workgroup of each pressure-force launch: per iteration, how many scenes go on
+// after it, then the step tag that marks the iteration's results as published.
 struct HostMirror {
-    unsigned long long max_dv[kMaxIterationsCap];
-    unsigned long long max_exp[kMaxIterationsCap];
+    unsigned int running[kMaxIterationsCap];
+    unsigned int tag[kMaxIterationsCap];
+};
+
+// Per-scene results, host-visible, rewritten at every iteration the scene runs, so
+// after the loop they hold the scene's last iteration.
+struct SceneResult {
+    unsigned long long max_dv;   // bits of the max density variation (precision R)
+    unsigned long long max_exp;  // bits of the max expansion
     unsigned long long candidates[2];
+    unsigned int iterations;
     unsigned int warm_rejected;
     unsigned int list_passes;
-    unsigned int tag[kMaxIterationsCap];
+    unsigned int pad;
 };
 
 // Inputs of the warm-start network, per fluid particle (8 floats, row-major):
@@ -97,8 +131,17 @@ struct HostMirror {
 struct FeatureArgs {
     float* feat;          // float[n][8] or null (no features)
     const uint4* counts;  // per-particle neighbour counts of the density pass
-    float p_ref;
-    float div_scale;      // m * 45 / (pi h^6) * dt * 100
+};
+
+// The batch as every step kernel sees it.
+template <typename R>
+struct Batch {
+    const SphParams<R>* scenes;   // [S]
+    const BlockRange* blocks;     // fluid ranges [0, fluid_blocks), then boundary ranges
+    LoopState* states;            // [S]
+    uint32_t scene_count;
+    uint32_t fluid_blocks, boundary_blocks;
+    uint32_t n_total, nb_total;   // list strides
 };
 
 template <typename R>
@@ -244,8 +287,7 @@ struct NeighborLists {
     uint32_t* bcount = nullptr;   // [n]
     uint32_t* bfcount = nullptr;  // [nb]
     uint32_t cap_f = 0, cap_b = 0, cap_bf = 0;
-    float r2 = 0.f;          // list radius squared; 0: no lists
-    float disp_limit = 0.f;  // largest max |xp - x| with which a pressure pass may use them
+    uint32_t on = 0;  // 0: no lists this step (every pass walks its buckets)
 };
 
 struct ListWriter {
@@ -273,16 +315,23 @@ __device__ __forceinline__ T block_reduce_sum(T v, T* lds) {
 }
 
 // ------------------------------------------------------------- kernels
+//
+// Every step kernel starts from its workgroup's BlockRange: the scene (whose
+// SphParams and LoopState it uses) and the range of particle indices it covers.
+// Indices are batch-wide; a scene's buckets are its own slice of the cell tables.
 
-// initialize (PCISPH/data/kernels/sort_utils.cl:12-23): key = hash(cell(pos)), src = id
+// initialize (PCISPH/data/kernels/sort_utils.cl:12-23): key = bucket_base +
+// hash(cell(pos)), src = index.  `blocks` is the fluid or the boundary part of the
+// block table.
 template <typename R>
-__global__ void k_hash(const V4<R>* __restrict__ pos, uint32_t n, R cell, HashGrid g,
-                       uint32_t* __restrict__ keys, uint32_t* __restrict__ src) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const V4<R> p = pos[i];
-    const int3 c = cell_of4<R>(p, cell);
-    keys[i] = hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, g);
+__global__ void k_hash(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
+                       const V4<R>* __restrict__ pos, uint32_t* __restrict__ keys, uint32_t* __restrict__ src) {
+    const BlockRange rg = blocks[blockIdx.x];
+    const uint32_t i = rg.begin + threadIdx.x;
+    if (i >= rg.end) return;
+    const SphParams<R>& P = scenes[rg.scene];
+    const int3 c = cell_of4<R>(pos[i], P.cell);
+    keys[i] = P.bucket_base + hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, P.grid);
     src[i] = i;
 }
 
@@ -312,6 +361,28 @@ __device__ __forceinline__ void wave_max_to(unsigned int* dst, float v) {
         b = o > b ? o : b;
     }
     if ((threadIdx.x & 63) == 0 && b) atomicMax(dst, b);
+}
+
+// Workgroup-wide maxima of up to four 64-bit values, folded into dst[q] by thread 0.
+template <int Q>
+__device__ __forceinline__ void block_max_to(unsigned long long (&m)[Q], unsigned long long* const (&dst)[Q],
+                                             unsigned long long (*lds)[8]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(m[q], off);
+            m[q] = o > m[q] ? o : m[q];
+        }
+        if (lane == 0) lds[q][wave] = m[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < Q) {
+        unsigned long long v = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = lds[threadIdx.x][w] > v ? lds[threadIdx.x][w] : v;
+        if (v) atomicMax(dst[threadIdx.x], v);
+    }
 }
 
 template <typename R>
@@ -384,65 +455,73 @@ __device__ __forceinline__ void walk(const uint32_t* __restrict__ list, uint32_t
 
 // update_density (pcisph.cl:65-92), plus per-particle candidate / neighbour counts,
 // plus this step's neighbour lists (every candidate within the list radius, in the
-// reference's bucket order).  Boundary particle pressures are reset here too
-// (boundary_pressure_initialization, pcisph.cl:114-119): threads n .. n+nb-1 handle
-// the boundary and record their fluid neighbours.
+// reference's bucket order).  Boundary ranges reset the boundary pressures
+// (boundary_pressure_initialization, pcisph.cl:114-119) and record their fluid
+// neighbours.
 template <typename R, bool FAST>
-__global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+__global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
                           const uint2* __restrict__ fcells, const V4<R>* __restrict__ pos,
                           V4<R>* __restrict__ xr, uint4* __restrict__ counts, R* __restrict__ bp,
-                          V4<R>* __restrict__ bxf, LoopState* __restrict__ st, NeighborLists NL) {
+                          V4<R>* __restrict__ bxf, NeighborLists NL) {
     __shared__ unsigned long long lds[16];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool lists = NL.r2 > 0.f;
-    const R lr2 = (R)NL.r2;
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<R> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
+    const uint2* fc = fcells + P.bucket_base;
+    const uint32_t i = rg.begin + threadIdx.x;
+    const bool in = i < rg.end;
+    const bool lists = NL.on != 0;
+    const R lr2 = (R)P.list_r2;
     unsigned long long cf_sum = 0, cb_sum = 0;
-    if (i < P.n) {
-        const V4<R> self = pos[i];
-        const int3 c = cell_of4<R>(self, P.cell);
-        R rho = 0;
-        uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
-        ListWriter wb{NL.b, P.n, i, NL.cap_b, 0}, wf{NL.f, P.n, i, NL.cap_f, 0};
-        if (P.nb)
-            walk_buckets<R>(bcells, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+    if (!(rg.flags & kRangeBoundary)) {
+        if (in) {
+            const uint2* bc = bcells + P.bucket_base;
+            const V4<R> self = pos[i];
+            const int3 c = cell_of4<R>(self, P.cell);
+            R rho = 0;
+            uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
+            ListWriter wb{NL.b, B.n_total, i, NL.cap_b, 0}, wf{NL.f, B.n_total, i, NL.cap_f, 0};
+            if (P.nb)
+                walk_buckets<R>(bc, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                                [&](const V4<R>& o, uint32_t j) {
+                                    R dx, dy, dz;
+                                    const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+                                    rho += poly6<R>(r2, P.h2);
+                                    ++cb;
+                                    nb += !(r2 > P.h2);
+                                    if (lists && r2 <= lr2) wb.push(j);
+                                });
+            walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& o, uint32_t j) {
                                 R dx, dy, dz;
                                 const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
                                 rho += poly6<R>(r2, P.h2);
-                                ++cb;
-                                nb += !(r2 > P.h2);
-                                if (lists && r2 <= lr2) wb.push(j);
+                                ++cf;
+                                nf += !(r2 > P.h2);
+                                if (lists && r2 <= lr2) wf.push(j);
                             });
-        walk_buckets<R>(fcells, c, P.grid, [&](uint32_t j) { return pos[j]; },
-                        [&](const V4<R>& o, uint32_t j) {
-                            R dx, dy, dz;
-                            const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
-                            rho += poly6<R>(r2, P.h2);
-                            ++cf;
-                            nf += !(r2 > P.h2);
-                            if (lists && r2 <= lr2) wf.push(j);
-                        });
-        rho *= P.density_scale;
-        V4<R> out = self;
-        out.w = rho;
-        xr[i] = out;
-        if (counts) counts[i] = make_uint4(cf, cb, nf, nb);
-        if (lists) {
-            NL.bcount[i] = wb.result();
-            NL.fcount[i] = wf.result();
+            rho *= P.density_scale;
+            V4<R> out = self;
+            out.w = rho;
+            xr[i] = out;
+            if (counts) counts[i] = make_uint4(cf, cb, nf, nb);
+            if (lists) {
+                NL.bcount[i] = P.nb ? wb.result() : 0u;
+                NL.fcount[i] = wf.result();
+            }
+            cf_sum = cf;
+            cb_sum = cb;
         }
-        cf_sum = cf;
-        cb_sum = cb;
-    } else if (i < P.n + P.nb) {
-        const uint32_t b = i - P.n;
+    } else if (in) {
+        const uint32_t b = i;
         bp[b] = 0;
         V4<R> o = bpos[b];
         o.w = 0;
         bxf[b] = o;
         if (lists) {
             const int3 c = cell_of4<R>(o, P.cell);
-            ListWriter w{NL.bf, P.nb, b, NL.cap_bf, 0};
-            walk_buckets<R>(fcells, c, P.grid, [&](uint32_t j) { return pos[j]; },
+            ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0};
+            walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& q, uint32_t j) {
                                 R dx, dy, dz;
                                 if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
@@ -453,19 +532,21 @@ __global__ void k_density(SphParams<R> P, const uint2* __restrict__ bcells, cons
     const unsigned long long a = block_reduce_sum(cf_sum, lds);
     __syncthreads();
     const unsigned long long b = block_reduce_sum(cb_sum, lds + 8);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && (a || b)) {
         atomicAdd(&st->fluid_candidates, a);
         atomicAdd(&st->boundary_candidates, b);
     }
 }
 
-// update_normal (pcisph.cl:94-112)
+// update_normal (pcisph.cl:94-112); fluid ranges only
 template <typename R, bool FAST>
-__global__ void k_normal(SphParams<R> P, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+__global__ void k_normal(Batch<R> B, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                          V4<R>* __restrict__ nrm, NeighborLists NL) {
     using O = Ops<R, FAST>;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const uint32_t i = rg.begin + threadIdx.x;
+    if (i >= rg.end) return;
+    const SphParams<R> P = B.scenes[rg.scene];
     const V4<R> self = xr[i];
     const int3 c = cell_of4<R>(self, P.cell);
     R nx = 0, ny = 0, nz = 0;
@@ -478,8 +559,8 @@ __global__ void k_normal(SphParams<R> P, const uint2* __restrict__ fcells, const
         ny += O::div(w * dy, o.w);
         nz += O::div(w * dz, o.w);
     };
-    const uint32_t len = NL.r2 > 0.f ? NL.fcount[i] : kListOverflow;
-    walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { return xr[j]; }, body);
+    const uint32_t len = NL.on ? NL.fcount[i] : kListOverflow;
+    walk<R>(NL.f, len, B.n_total, i, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xr[j]; }, body);
     V4<R> out;
     out.x = nx * P.normal_scale;
     out.y = ny * P.normal_scale;
@@ -505,18 +586,21 @@ __device__ __forceinline__ void integrate(const SphParams<R>& P, const V4<R>& x,
 }
 
 // force_initialization (pcisph.cl:121-173) fused with the first prediction of the
-// correction loop (pcisph.cl:175-193 with F_p = 0, Fluid.cpp:253-260).
+// correction loop (pcisph.cl:175-193 with F_p = 0, Fluid.cpp:253-260); fluid ranges.
 // With features requested it also gathers the warm-start network's inputs; the
 // velocity's w component carries the previous step's converged pressure.
 template <typename R, bool FAST>
-__global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+__global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                              const V4<R>* __restrict__ vel, const V4<R>* __restrict__ nrm,
                              V4<R>* __restrict__ fo, R* __restrict__ p, V4<R>* __restrict__ fp,
-                             V4<R>* __restrict__ xp, FeatureArgs F, NeighborLists NL, LoopState* __restrict__ st) {
+                             V4<R>* __restrict__ xp, FeatureArgs F, NeighborLists NL) {
     using O = Ops<R, FAST>;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<R> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
+    const uint32_t i = rg.begin + threadIdx.x;
     float moved = 0.f;
-    if (i < P.n) {
+    if (i < rg.end) {
         const V4<R> self = xr[i];
         const V4<R> sv = vel[i];
         const V4<R> sn = nrm[i];
@@ -564,8 +648,8 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
                 uz += corr * (sn.z - on.z);
             }
         };
-        const uint32_t len = NL.r2 > 0.f ? NL.fcount[i] : kListOverflow;
-        walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, load, body);
+        const uint32_t len = NL.on ? NL.fcount[i] : kListOverflow;
+        walk<R>(NL.f, len, B.n_total, i, fcells + P.bucket_base, c, P.grid, load, body);
         const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
         const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
         const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
@@ -589,78 +673,84 @@ __global__ void k_force_init(SphParams<R> P, const uint2* __restrict__ fcells, c
             const float inv = a_n > 0.f ? 1.f / a_n : 0.f;
             const float speed = sqrtf((float)(sv.x * sv.x + sv.y * sv.y + sv.z * sv.z));
             float4* out = reinterpret_cast<float4*>(F.feat + (size_t)i * 8);
-            out[0] = make_float4(((float)self.w / rho0 - 1.f) * 100.f, (float)cnt.z / 32.f, (float)sv.w / F.p_ref,
+            out[0] = make_float4(((float)self.w / rho0 - 1.f) * 100.f, (float)cnt.z / 32.f, (float)sv.w / P.p_ref,
                                  speed * (float)P.dt / (float)P.h * 100.f);
-            out[1] = make_float4((a_rho * inv / rho0 - 1.f) * 100.f, a_p * inv / F.p_ref, (float)cnt.w / 32.f,
-                                 a_div * F.div_scale);
+            out[1] = make_float4((a_rho * inv / rho0 - 1.f) * 100.f, a_p * inv / P.p_ref, (float)cnt.w / 32.f,
+                                 a_div * P.div_scale);
         }
     }
     wave_max_to(&st->max_disp[0], moved);
 }
 
-// update_pressure (pcisph.cl:212-263): fluid mode for threads 0..n-1, boundary mode
-// for threads n..n+nb-1 (the two launches of Fluid.cpp:263-280 are independent).
+// update_pressure (pcisph.cl:212-263): fluid mode on fluid ranges, boundary mode on
+// boundary ranges (the two launches of Fluid.cpp:263-280 are independent).
 // Also publishes x + p/rho^2 for the force pass and the per-iteration max dv.
 // Distances are taken at the predicted positions, so the step's lists (built at
 // x with radius (1+skin)h) hold every pair within h only while no predicted
-// position has moved more than skin*h/2 from x; past NL.disp_limit the pass walks
-// the buckets instead.
+// position of the scene has moved more than skin*h/2 from x; past P.disp_limit
+// the pass walks the buckets instead.
 template <typename R, bool FAST>
-__global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict__ st,
-                           const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
-                           const R* __restrict__ binit, const uint2* __restrict__ fcells,
-                           const V4<R>* __restrict__ xr, const V4<R>* __restrict__ xp, R* __restrict__ dv,
-                           R* __restrict__ p, V4<R>* __restrict__ xf, R* __restrict__ bp,
-                           V4<R>* __restrict__ bxf, NeighborLists NL) {
+__global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
+                           const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
+                           const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+                           const V4<R>* __restrict__ xp, R* __restrict__ dv, R* __restrict__ p,
+                           V4<R>* __restrict__ xf, R* __restrict__ bp, V4<R>* __restrict__ bxf, NeighborLists NL) {
     using O = Ops<R, FAST>;
-    __shared__ unsigned long long lds[16];
+    __shared__ unsigned long long lds[2][8];
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<R> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
     if (iteration_skipped<R>(st, iter, P)) return;
-    const bool lists = NL.r2 > 0.f && __uint_as_float(st->max_disp[iter]) <= NL.disp_limit;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && lists) atomicAdd(&st->list_passes, 1u);
-    unsigned long long my_max = 0, my_exp = 0;
-    if (i < P.n) {
-        const V4<R> self = xr[i];
-        const V4<R> pr = xp[i];
-        const int3 c = cell_of4<R>(self, P.cell);
-        R pd = 0;
-        if (P.nb) {
-            const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
-            walk<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+    const bool lists = NL.on && __uint_as_float(st->max_disp[iter]) <= P.disp_limit;
+    if (threadIdx.x == 0 && (rg.flags & kRangeFirstOfScene) && lists) atomicAdd(&st->list_passes, 1u);
+    const uint2* fc = fcells + P.bucket_base;
+    const uint32_t i = rg.begin + threadIdx.x;
+    unsigned long long m[2] = {0, 0};  // max dv, max expansion
+    if (!(rg.flags & kRangeBoundary)) {
+        if (i < rg.end) {
+            const V4<R> self = xr[i];
+            const V4<R> pr = xp[i];
+            const int3 c = cell_of4<R>(self, P.cell);
+            R pd = 0;
+            if (P.nb) {
+                const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
+                walk<R>(NL.b, len, B.n_total, i, bcells + P.bucket_base, c, P.grid,
+                        [&](uint32_t j) { return bpos[j]; },
+                        [&](const V4<R>& o, uint32_t) {
+                            R dx, dy, dz;
+                            pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                        });
+            }
+            const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
+            walk<R>(NL.f, len, B.n_total, i, fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
                     [&](const V4<R>& o, uint32_t) {
                         R dx, dy, dz;
                         pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
                     });
+            pd *= P.density_scale;
+            const R v = fmax(R(0), pd - P.rho0);
+            // expansion counts only where the step began at or above rest density: free-
+            // surface particles sit below rho0 whatever the pressure
+            if (self.w >= P.rho0) m[1] = Bits<R>::of(fmax(R(0), P.rho0 - pd));
+            dv[i] = v;
+            R pi = p[i];
+            if (v != R(0)) {
+                pi += v * P.delta;
+                p[i] = pi;
+            }
+            V4<R> out = self;
+            out.w = O::div(pi, self.w * self.w);
+            xf[i] = out;
+            m[0] = Bits<R>::of(v);
         }
-        const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
-        walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { return xp[j]; },
-                [&](const V4<R>& o, uint32_t) {
-                    R dx, dy, dz;
-                    pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-                });
-        pd *= P.density_scale;
-        const R v = fmax(R(0), pd - P.rho0);
-        // expansion counts only where the step began at or above rest density: free-
-        // surface particles sit below rho0 whatever the pressure
-        if (self.w >= P.rho0) my_exp = Bits<R>::of(fmax(R(0), P.rho0 - pd));
-        dv[i] = v;
-        R pi = p[i];
-        if (v != R(0)) {
-            pi += v * P.delta;
-            p[i] = pi;
-        }
-        V4<R> out = self;
-        out.w = O::div(pi, self.w * self.w);
-        xf[i] = out;
-        my_max = Bits<R>::of(v);
-    } else if (i < P.n + P.nb) {
-        const uint32_t b = i - P.n;
+    } else if (i < rg.end) {
+        const uint32_t b = i;
         const V4<R> self = bpos[b];
         const int3 c = cell_of4<R>(self, P.cell);
         R pd = 0;
         pd += binit[b];
         const uint32_t len = lists ? NL.bfcount[b] : kListOverflow;
-        walk<R>(NL.bf, len, P.nb, b, fcells, c, P.grid, [&](uint32_t j) { return xp[j]; },
+        walk<R>(NL.bf, len, B.nb_total, b, fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
                 [&](const V4<R>& o, uint32_t) {
                     R dx, dy, dz;
                     pd += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
@@ -675,64 +765,77 @@ __global__ void k_pressure(SphParams<R> P, uint32_t iter, LoopState* __restrict_
             bxf[b] = out;
         }
     }
+    unsigned long long* const dst[2] = {&st->max_dv[iter], &st->max_exp[iter]};
+    block_max_to<2>(m, dst, lds);
+}
+
+// Publication of iteration `iter` by the first workgroup of the pressure-force launch
+// (the pressure pass that produced the results has completed): every scene that ran
+// the iteration writes its results, the number of scenes that go on is stored, and
+// then the step tag, so the host decides how far to enqueue without waiting on the
+// stream.
+template <typename R>
+__device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* host, SceneResult* res,
+                                  unsigned int tag) {
+    __shared__ unsigned int s_run[16];
+    unsigned int running = 0;
+    for (uint32_t s = threadIdx.x; s < B.scene_count; s += blockDim.x) {
+        const SphParams<R>& P = B.scenes[s];
+        const LoopState* st = B.states + s;
+        if (P.n == 0 || iteration_skipped<R>(st, iter, P)) continue;
+        SceneResult r;
+        r.max_dv = st->max_dv[iter];
+        r.max_exp = st->max_exp[iter];
+        r.candidates[0] = st->fluid_candidates;
+        r.candidates[1] = st->boundary_candidates;
+        r.iterations = iter + 1;
+        r.warm_rejected = st->warm_rejected;
+        r.list_passes = st->list_passes;
+        r.pad = 0;
+        res[s] = r;
+        if (!iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations, P.rho0, P.threshold))
+            ++running;
+    }
+    __threadfence_system();
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(my_max, off);
-        my_max = o > my_max ? o : my_max;
-        const unsigned long long e = __shfl_xor(my_exp, off);
-        my_exp = e > my_exp ? e : my_exp;
-    }
+    for (int off = 32; off > 0; off >>= 1) running += __shfl_xor(running, off);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        lds[wave] = my_max;
-        lds[8 + wave] = my_exp;
-    }
+    if (lane == 0) s_run[wave] = running;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long m = 0, e = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-            m = lds[w] > m ? lds[w] : m;
-            e = lds[8 + w] > e ? lds[8 + w] : e;
-        }
-        if (m) atomicMax(&st->max_dv[iter], m);
-        if (e) atomicMax(&st->max_exp[iter], e);
+        unsigned int total = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += s_run[w];
+        host->running[iter] = total;
+        __threadfence_system();
+        __hip_atomic_store(&host->tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
 // update_pressure_force (pcisph.cl:265-300) fused with what follows it: the next
 // prediction (pcisph.cl:175-193 into xp, Fluid.cpp:253-260) or, when this iteration
-// ends the loop, the final in-place integration (Fluid.cpp:309-316) into pos_out/vel_out.
+// ends the scene's loop, the final in-place integration (Fluid.cpp:309-316) into
+// pos_out/vel_out.  Fluid ranges only.
 template <typename R, bool FAST>
-__global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __restrict__ st,
-                                 const uint2* __restrict__ bcells, const V4<R>* __restrict__ bxf,
-                                 const uint2* __restrict__ fcells, const V4<R>* __restrict__ xf,
-                                 const V4<R>* __restrict__ fo, const V4<R>* __restrict__ vel,
-                                 V4<R>* __restrict__ fp, V4<R>* __restrict__ xp, V4<R>* __restrict__ pos_out,
-                                 V4<R>* __restrict__ vel_out, const R* __restrict__ p, HostMirror* host,
-                                 unsigned int tag, int warm_pass, V4<R>* __restrict__ xp_save,
-                                 NeighborLists NL) {
+__global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
+                                 const V4<R>* __restrict__ bxf, const uint2* __restrict__ fcells,
+                                 const V4<R>* __restrict__ xf, const V4<R>* __restrict__ fo,
+                                 const V4<R>* __restrict__ vel, V4<R>* __restrict__ fp, V4<R>* __restrict__ xp,
+                                 V4<R>* __restrict__ pos_out, V4<R>* __restrict__ vel_out,
+                                 const R* __restrict__ p, HostMirror* host, SceneResult* res, unsigned int tag,
+                                 int warm_pass, V4<R>* __restrict__ xp_save, NeighborLists NL) {
     using O = Ops<R, FAST>;
     // warm_pass: the pressure force of the network's initial pressures, before the
     // first prediction -- not an iteration of the loop.
+    if (blockIdx.x == 0 && !warm_pass) publish_iteration<R>(B, iter, host, res, tag);
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<R> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
     if (!warm_pass && iteration_skipped<R>(st, iter, P)) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && !warm_pass) {
-        // Publish this iteration's results to host-visible memory (the pressure pass
-        // that produced them has completed): the host decides how far to enqueue
-        // without waiting on the stream.  Values first, then the step tag.
-        host->max_dv[iter] = st->max_dv[iter];
-        host->max_exp[iter] = st->max_exp[iter];
-        host->candidates[0] = st->fluid_candidates;
-        host->candidates[1] = st->boundary_candidates;
-        host->warm_rejected = st->warm_rejected;
-        host->list_passes = st->list_passes;
-        __threadfence_system();
-        __hip_atomic_store(&host->tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
     const bool last = !warm_pass && iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations,
                                                          P.rho0, P.threshold);
+    const uint32_t i = rg.begin + threadIdx.x;
     float moved = 0.f;
-    if (i < P.n) {
+    if (i < rg.end) {
         const V4<R> self = xf[i];
         const R sf = self.w;
         const int3 c = cell_of4<R>(self, P.cell);
@@ -750,14 +853,14 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
             fz += O::div(dz * dr * dr, r) * factor;
         };
         // positions here are x, where the lists are exact
-        const bool lists = NL.r2 > 0.f;
+        const bool lists = NL.on != 0;
         if (P.nb) {
             const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
-            walk<R>(NL.b, len, P.n, i, bcells, c, P.grid, [&](uint32_t j) { return bxf[j]; },
-                    [&](const V4<R>& o, uint32_t) { add(o); });
+            walk<R>(NL.b, len, B.n_total, i, bcells + P.bucket_base, c, P.grid,
+                    [&](uint32_t j) { return bxf[j]; }, [&](const V4<R>& o, uint32_t) { add(o); });
         }
         const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
-        walk<R>(NL.f, len, P.n, i, fcells, c, P.grid, [&](uint32_t j) { return xf[j]; },
+        walk<R>(NL.f, len, B.n_total, i, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xf[j]; },
                 [&](const V4<R>& o, uint32_t) { add(o); });
         V4<R> f;
         f.x = fx * P.pf_scale;
@@ -785,75 +888,101 @@ __global__ void k_pressure_force(SphParams<R> P, uint32_t iter, LoopState* __res
     if (!last) wave_max_to(&st->max_disp[warm_pass ? 0 : iter + 1], moved);
 }
 
+// The network's output (pressure in units of p_ref, written by the inference kernel)
+// made a pressure and the force pass's p/rho^2; fluid ranges.
+template <typename R>
+__global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, R* __restrict__ p, V4<R>* __restrict__ xf) {
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const uint32_t i = rg.begin + threadIdx.x;
+    if (i >= rg.end) return;
+    const float p_ref = B.scenes[rg.scene].p_ref;
+    const R pi = (R)((float)p[i] * p_ref);
+    p[i] = pi;
+    V4<R> o = xr[i];
+    o.w = pi / (o.w * o.w);
+    xf[i] = o;
+}
+
 // First iteration of a warm-started step: the pressure update of pcisph.cl:212-263
 // for two starts at once -- the network's (xp, p) and the plain one (xp0, p = 0) --
-// sharing the candidate loop.  The step continues with whichever start has the
+// sharing the candidate loop.  Each scene continues with whichever start has the
 // smaller residual max |rho* - rho0| (k_select_start), so a prediction that makes
 // the residual worse is dropped and the loop is exactly the plain one.
 template <typename R, bool FAST>
-__global__ void k_pressure_dual(SphParams<R> P, LoopState* __restrict__ st, const uint2* __restrict__ bcells,
-                                const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
-                                const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
-                                const V4<R>* __restrict__ xp, R* __restrict__ dv, R* __restrict__ p,
-                                V4<R>* __restrict__ xf, R* __restrict__ bp, V4<R>* __restrict__ bxf,
-                                DualArrays<R> D) {
+__global__ void k_pressure_dual(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+                                const R* __restrict__ binit, const uint2* __restrict__ fcells,
+                                const V4<R>* __restrict__ xr, const V4<R>* __restrict__ xp, R* __restrict__ dv,
+                                R* __restrict__ p, V4<R>* __restrict__ xf, R* __restrict__ bp,
+                                V4<R>* __restrict__ bxf, DualArrays<R> D) {
     using O = Ops<R, FAST>;
     __shared__ unsigned long long lds[4][8];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<R> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
+    const uint2* fc = fcells + P.bucket_base;
+    const uint32_t i = rg.begin + threadIdx.x;
     unsigned long long m[4] = {0, 0, 0, 0};  // dv warm, exp warm, dv plain, exp plain
-    if (i < P.n) {
-        const V4<R> self = xr[i];
-        const V4<R> pw = xp[i], pz = D.xp0[i];
-        const int3 c = cell_of4<R>(self, P.cell);
-        R sw = 0, sz = 0;
-        if (P.nb)
-            for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
-                const V4<R> o = bpos[j];
-                R dx, dy, dz;
-                sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-                sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-            });
-        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
-            const V4<R> ow = xp[j], oz = D.xp0[j];
-            R dx, dy, dz;
-            sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, ow.x, ow.y, ow.z, dx, dy, dz), P.h2);
-            sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, oz.x, oz.y, oz.z, dx, dy, dz), P.h2);
-        });
-        sw *= P.density_scale;
-        sz *= P.density_scale;
-        const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
-        m[0] = Bits<R>::of(vw);
-        const bool bulk = self.w >= P.rho0;  // as in k_pressure: no free-surface particles
-        m[1] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sw)) : 0ull;
-        m[2] = Bits<R>::of(vz);
-        m[3] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sz)) : 0ull;
-        dv[i] = vw;
-        D.dv0[i] = vz;
-        R pi = p[i];
-        if (vw != R(0)) {
-            pi += vw * P.delta;
-            p[i] = pi;
+    if (!(rg.flags & kRangeBoundary)) {
+        if (i < rg.end) {
+            const V4<R> self = xr[i];
+            const V4<R> pw = xp[i], pz = D.xp0[i];
+            const int3 c = cell_of4<R>(self, P.cell);
+            R sw = 0, sz = 0;
+            if (P.nb)
+                walk_buckets<R>(bcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                                [&](const V4<R>& o, uint32_t) {
+                                    R dx, dy, dz;
+                                    sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                                    sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                                });
+            struct Pair {
+                V4<R> w, z;
+            };
+            walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
+                            [&](const Pair& o, uint32_t) {
+                                R dx, dy, dz;
+                                sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.w.x, o.w.y, o.w.z, dx, dy, dz), P.h2);
+                                sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, o.z.x, o.z.y, o.z.z, dx, dy, dz), P.h2);
+                            });
+            sw *= P.density_scale;
+            sz *= P.density_scale;
+            const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
+            m[0] = Bits<R>::of(vw);
+            const bool bulk = self.w >= P.rho0;  // as in k_pressure: no free-surface particles
+            m[1] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sw)) : 0ull;
+            m[2] = Bits<R>::of(vz);
+            m[3] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sz)) : 0ull;
+            dv[i] = vw;
+            D.dv0[i] = vz;
+            R pi = p[i];
+            if (vw != R(0)) {
+                pi += vw * P.delta;
+                p[i] = pi;
+            }
+            const R pz0 = vz != R(0) ? R(0) + vz * P.delta : R(0);
+            D.p0[i] = pz0;
+            V4<R> out = self;
+            out.w = O::div(pi, self.w * self.w);
+            xf[i] = out;
+            out.w = O::div(pz0, self.w * self.w);
+            D.xf0[i] = out;
         }
-        const R pz0 = vz != R(0) ? R(0) + vz * P.delta : R(0);
-        D.p0[i] = pz0;
-        V4<R> out = self;
-        out.w = O::div(pi, self.w * self.w);
-        xf[i] = out;
-        out.w = O::div(pz0, self.w * self.w);
-        D.xf0[i] = out;
-    } else if (i < P.n + P.nb) {
-        const uint32_t b = i - P.n;
+    } else if (i < rg.end) {
+        const uint32_t b = i;
         const V4<R> self = bpos[b];
         const int3 c = cell_of4<R>(self, P.cell);
         R sw = 0, sz = 0;
         sw += binit[b];
         sz += binit[b];
-        for_each_candidate<R>(fcells, c, P.grid, [&](uint32_t j) {
-            const V4<R> ow = xp[j], oz = D.xp0[j];
-            R dx, dy, dz;
-            sw += poly6<R>(dist2<R>(self.x, self.y, self.z, ow.x, ow.y, ow.z, dx, dy, dz), P.h2);
-            sz += poly6<R>(dist2<R>(self.x, self.y, self.z, oz.x, oz.y, oz.z, dx, dy, dz), P.h2);
-        });
+        struct Pair {
+            V4<R> w, z;
+        };
+        walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
+                        [&](const Pair& o, uint32_t) {
+                            R dx, dy, dz;
+                            sw += poly6<R>(dist2<R>(self.x, self.y, self.z, o.w.x, o.w.y, o.w.z, dx, dy, dz), P.h2);
+                            sz += poly6<R>(dist2<R>(self.x, self.y, self.z, o.z.x, o.z.y, o.z.z, dx, dy, dz), P.h2);
+                        });
         sw *= P.density_scale;
         sz *= P.density_scale;
         const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
@@ -867,69 +996,62 @@ __global__ void k_pressure_dual(SphParams<R> P, LoopState* __restrict__ st, cons
         out.w = O::div(pbz, P.rho0_sq);
         D.bxf0[b] = out;
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const unsigned long long o = __shfl_xor(m[q], off);
-            m[q] = o > m[q] ? o : m[q];
-        }
-        if (lane == 0) lds[q][wave] = m[q];
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        unsigned long long v = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = lds[threadIdx.x][w] > v ? lds[threadIdx.x][w] : v;
-        // 0: dv warm, 1: expansion warm, 2: dv plain, 3: expansion plain
-        unsigned long long* dst[4] = {&st->dual_dv[0], &st->dual_exp[0], &st->dual_dv[1], &st->dual_exp[1]};
-        if (v) atomicMax(dst[threadIdx.x], v);
+    // 0: dv warm, 1: expansion warm, 2: dv plain, 3: expansion plain
+    unsigned long long* const dst[4] = {&st->dual_dv[0], &st->dual_exp[0], &st->dual_dv[1], &st->dual_exp[1]};
+    block_max_to<4>(m, dst, lds);
+}
+
+// Per scene: picks the start with the smaller residual and records its iteration-0
+// results where the loop's termination logic reads them.
+template <typename R>
+__global__ void k_select_start(LoopState* states, uint32_t scene_count) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < scene_count; s += gridDim.x * blockDim.x) {
+        LoopState* st = states + s;
+        const R dw = Bits<R>::to(st->dual_dv[0]), ew = Bits<R>::to(st->dual_exp[0]);
+        const R dz = Bits<R>::to(st->dual_dv[1]), ez = Bits<R>::to(st->dual_exp[1]);
+        const R rw = dw > ew ? dw : ew, rz = dz > ez ? dz : ez;
+        const bool reject = !(rw < rz);
+        st->warm_rejected = reject ? 1u : 0u;
+        st->max_dv[0] = reject ? st->dual_dv[1] : st->dual_dv[0];
+        st->max_exp[0] = reject ? st->dual_exp[1] : st->dual_exp[0];
     }
 }
 
-// Picks the start with the smaller residual and records its iteration-0 results
-// where the loop's termination logic reads them.
+// After k_select_start rejected the network's start of a scene: make the plain
+// start's arrays the live ones (fluid and boundary ranges).
 template <typename R>
-__global__ void k_select_start(LoopState* st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const R dw = Bits<R>::to(st->dual_dv[0]), ew = Bits<R>::to(st->dual_exp[0]);
-    const R dz = Bits<R>::to(st->dual_dv[1]), ez = Bits<R>::to(st->dual_exp[1]);
-    const R rw = dw > ew ? dw : ew, rz = dz > ez ? dz : ez;
-    const bool reject = !(rw < rz);
-    st->warm_rejected = reject ? 1u : 0u;
-    st->max_dv[0] = reject ? st->dual_dv[1] : st->dual_dv[0];
-    st->max_exp[0] = reject ? st->dual_exp[1] : st->dual_exp[0];
-}
-
-// After k_select_start rejected the network's start: make the plain start's
-// arrays the live ones (threads 0..n-1 fluid, n..n+nb-1 boundary).
-template <typename R>
-__global__ void k_adopt_plain_start(uint32_t n, uint32_t nb, const LoopState* __restrict__ st, DualArrays<R> D,
-                                    V4<R>* __restrict__ xp, V4<R>* __restrict__ xf, R* __restrict__ p,
-                                    R* __restrict__ dv, R* __restrict__ bp, V4<R>* __restrict__ bxf) {
-    if (!st->warm_rejected) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+__global__ void k_adopt_plain_start(Batch<R> B, DualArrays<R> D, V4<R>* __restrict__ xp, V4<R>* __restrict__ xf,
+                                    R* __restrict__ p, R* __restrict__ dv, R* __restrict__ bp,
+                                    V4<R>* __restrict__ bxf) {
+    const BlockRange rg = B.blocks[blockIdx.x];
+    if (!B.states[rg.scene].warm_rejected) return;
+    const uint32_t i = rg.begin + threadIdx.x;
+    if (i >= rg.end) return;
+    if (!(rg.flags & kRangeBoundary)) {
         xp[i] = D.xp0[i];
         xf[i] = D.xf0[i];
         p[i] = D.p0[i];
         dv[i] = D.dv0[i];
-    } else if (i < n + nb) {
-        bp[i - n] = D.bp0[i - n];
-        bxf[i - n] = D.bxf0[i - n];
+    } else {
+        bp[i] = D.bp0[i];
+        bxf[i] = D.bxf0[i];
     }
 }
 
-// initialize_boundary_boundary_pred_densities (pcisph.cl:195-210), opt-in.
+// initialize_boundary_boundary_pred_densities (pcisph.cl:195-210), opt-in; `blocks`
+// is the boundary part of the block table.
 template <typename R, bool FAST>
-__global__ void k_boundary_init(SphParams<R> P, const uint2* __restrict__ bcells,
-                                const V4<R>* __restrict__ bpos, R* __restrict__ binit) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= P.nb) return;
+__global__ void k_boundary_init(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
+                                const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+                                R* __restrict__ binit) {
+    const BlockRange rg = blocks[blockIdx.x];
+    const uint32_t b = rg.begin + threadIdx.x;
+    if (b >= rg.end) return;
+    const SphParams<R> P = scenes[rg.scene];
     const V4<R> self = bpos[b];
     const int3 c = cell_of4<R>(self, P.cell);
     R r = 0;
-    for_each_candidate<R>(bcells, c, P.grid, [&](uint32_t j) {
+    for_each_candidate<R>(bcells + P.bucket_base, c, P.grid, [&](uint32_t j) {
         const V4<R> o = bpos[j];
         R dx, dy, dz;
         r += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);

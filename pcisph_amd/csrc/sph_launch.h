@@ -10,11 +10,13 @@
 
 namespace pcisph {
 
-// Device arrays of one solver (precision R).  Fluid arrays hold N elements,
-// boundary arrays Nb, cell tables B (uint2 start/end).
+// Device arrays of one batch of scenes (precision R).  Fluid arrays hold the
+// batch's N fluid particles, boundary arrays its Nb boundary particles, cell tables
+// its B buckets (uint2 start/end); `batch` holds the scene and block tables.
 template <typename R>
 struct DeviceArrays {
     uint32_t n = 0, nb = 0, B = 0;
+    Batch<R> batch{};
     // fluid: public state in the previous step's order, and this step's sorted copies
     V4<R>* pos_in = nullptr;
     V4<R>* vel_in = nullptr;
@@ -41,36 +43,38 @@ struct DeviceArrays {
     uint32_t* bkeys[2] = {nullptr, nullptr};
     uint32_t* bvals[2] = {nullptr, nullptr};
     uint2* bcells = nullptr;
-    LoopState* state = nullptr;
     NeighborLists NL;  // r2 == 0: no lists this step
 };
 
+// `block` must be the workgroup size the batch's block table was built for.
 struct Launch {
     hipStream_t stream;
     uint32_t block;
 };
 
+// Each launch covers the whole batch: fluid-only kernels run the fluid ranges of the
+// block table, the others fluid and boundary ranges.
 template <typename R, bool FAST>
 struct SphLaunch {
-    static void density(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
-    static void normal(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
-    static void force_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
-                           const FeatureArgs& F);
-    static void pressure(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A, uint32_t iter);
-    static void pressure_force(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
-                               uint32_t iter, HostMirror* host, unsigned int tag, int warm_pass,
-                               V4<R>* xp_save);
+    static void density(const Launch& L, const DeviceArrays<R>& A);
+    static void normal(const Launch& L, const DeviceArrays<R>& A);
+    static void force_init(const Launch& L, const DeviceArrays<R>& A, const FeatureArgs& F);
+    static void pressure(const Launch& L, const DeviceArrays<R>& A, uint32_t iter);
+    static void pressure_force(const Launch& L, const DeviceArrays<R>& A, uint32_t iter, HostMirror* host,
+                               SceneResult* res, unsigned int tag, int warm_pass, V4<R>* xp_save);
     // iteration 0 of a warm-started step (k_pressure_dual, k_select_start, k_adopt_plain_start)
-    static void pressure_dual(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A,
-                              const DualArrays<R>& D);
-    static void boundary_init(const Launch& L, const SphParams<R>& P, const DeviceArrays<R>& A);
+    static void pressure_dual(const Launch& L, const DeviceArrays<R>& A, const DualArrays<R>& D);
+    static void boundary_init(const Launch& L, const DeviceArrays<R>& A);
 };
 
 // Precision-only (no arithmetic mode) helpers, instantiated in sph_ieee.hip.
 template <typename R>
 struct DataLaunch {
-    static void hash(const Launch& L, const V4<R>* pos, uint32_t n, R cell, const HashGrid& g,
-                     uint32_t* keys, uint32_t* vals);
+    // keys = bucket_base + hash of every fluid (boundary = false) or boundary particle
+    static void hash(const Launch& L, const Batch<R>& B, bool boundary, const V4<R>* pos, uint32_t* keys,
+                     uint32_t* vals);
+    // p = network output * p_ref of the scene, xf = (x, p / rho^2)
+    static void warm_apply(const Launch& L, const DeviceArrays<R>& A);
     static void reorder(const Launch& L, uint32_t n, const uint32_t* keys, const uint32_t* src,
                         const V4<R>* pin, const V4<R>* vin, V4<R>* pout, V4<R>* vout, uint2* cells);
     // float[3n] <-> V4<R> (w set to 0 on the way in)
@@ -86,6 +90,8 @@ struct DataLaunch {
                                 uint32_t offset);
     // p[i].xyz += (dx, dy, dz) in precision R
     static void translate(const Launch& L, V4<R>* p, uint32_t n, float dx, float dy, float dz);
+    // every scene's boundary positions += shift[3 s .. 3 s + 2] (device array)
+    static void translate_scenes(const Launch& L, const Batch<R>& B, const float* shift, V4<R>* bpos);
 };
 
 void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);

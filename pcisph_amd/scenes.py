@@ -168,3 +168,85 @@ def small_dam_break(block=(8, 8, 8), tank=(0.4, 0.4, 0.3), spacing=0.02, jitter=
                     seed=1234, steps=5) -> Scene:
     """A few hundred particles: sizes the oracle finishes in well under a second."""
     return dam_break(block, spacing, jitter, tank, seed=seed, name="small_dam_break", steps=steps)
+
+
+def sloshing_tank(amplitude: float = 0.5, frequency: float = 0.8, base: Scene | None = None,
+                  name: str = "sloshing_tank") -> Scene:
+    """`base` (default: the 1,048,576-particle dam break) in a tank that oscillates
+    along x as amplitude * sin(2 pi frequency t); boundary_shift() gives each step's
+    displacement of the walls."""
+    sc = base if base is not None else dam_break_1m()
+    sc.name = name
+    sc.meta = dict(sc.meta, slosh={"amplitude": amplitude, "frequency": frequency})
+    return sc
+
+
+def boundary_shift(scene: Scene, step: int) -> tuple[float, float, float]:
+    """Wall displacement from the start of step `step` to the start of step + 1 for a
+    sloshing scene, (0, 0, 0) otherwise."""
+    slosh = scene.meta.get("slosh")
+    if not slosh:
+        return (0.0, 0.0, 0.0)
+    a, f, dt = slosh["amplitude"], slosh["frequency"], scene.delta_t
+    w = 2.0 * np.pi * f
+    return (a * (np.sin(w * (step + 1) * dt) - np.sin(w * step * dt)), 0.0, 0.0)
+
+
+def log_uniform_counts(count: int, n_min: int, n_max: int, seed: int) -> np.ndarray:
+    """int64[count] particle counts log-uniform in [n_min, n_max] (PCG32 stream per scene)."""
+    u = (pcg32_streams(seed, count, 1)[:, 0] >> np.uint32(8)).astype(np.float64) / (1 << 24)
+    return np.rint(np.exp(np.log(n_min) + u * (np.log(n_max) - np.log(n_min)))).astype(np.int64)
+
+
+def scaled_dam_break(target: int, seed: int, name: str = "dam_break") -> Scene:
+    """About `target` particles: an (n, 2n, n) block 0.4 x 0.8 x 0.4 m in a 1 x 1 x 1 m
+    tank, the spacing set by the count (jitter 5% of it)."""
+    n = max(2, int(round((target / 2.0) ** (1.0 / 3.0))))
+    spacing = 0.4 / n
+    cells = int(round(1.0 / spacing))
+    return dam_break((n, 2 * n, n), spacing, 0.05 * spacing, (cells * spacing,) * 3, seed=seed, name=name)
+
+
+def scaled_sphere_drop(target: int, seed: int, name: str = "sphere_drop") -> Scene:
+    """About `target` particles: a 0.1 m deep still pool on the floor of a 1 x 1 x 1 m
+    tank and a sphere of radius 0.15 m whose bottom is 0.3 m above it, the spacing set
+    by the count."""
+    volume = 1.0 * 0.1 * 1.0 + 4.0 / 3.0 * np.pi * 0.15 ** 3
+    spacing = (volume / target) ** (1.0 / 3.0)
+    cells = int(round(1.0 / spacing))
+    depth = max(1, int(round(0.1 / spacing)))
+    ball = int(round(4.0 / 3.0 * np.pi * (0.15 / spacing) ** 3))
+    return sphere_drop(ball, depth, spacing, (cells, cells, cells), drop_height=0.3, seed=seed, name=name)
+
+
+def batch_mix(count: int, n_min: int, n_max: int, seed: int,
+              kinds=("dam_break", "sphere_drop")) -> list[Scene]:
+    """`count` independent scenes with particle counts log-uniform in [n_min, n_max],
+    kinds assigned in turn: "dam_break", "sphere_drop", "sloshing" (a dam break whose
+    tank oscillates along x with 0.1 m amplitude at 0.8 Hz)."""
+    targets = log_uniform_counts(count, n_min, n_max, seed)
+    out = []
+    for k, t in enumerate(targets):
+        kind = kinds[k % len(kinds)]
+        if kind == "dam_break":
+            sc = scaled_dam_break(int(t), seed + k, name=f"dam_break_{k}")
+        elif kind == "sphere_drop":
+            sc = scaled_sphere_drop(int(t), seed + k, name=f"sphere_drop_{k}")
+        elif kind == "sloshing":
+            sc = sloshing_tank(0.1, 0.8, scaled_dam_break(int(t), seed + k), name=f"sloshing_{k}")
+        else:
+            raise ValueError(f"unknown scene kind {kind}")
+        out.append(sc)
+    return out
+
+
+def batch_64(seed: int = 20240612) -> list[Scene]:
+    """64 scenes, counts log-uniform in [2,048, 262,144], half dam breaks and half
+    sphere drops, 500 steps."""
+    return batch_mix(64, 2048, 262144, seed)
+
+
+def batch_256(seed: int = 20240613) -> list[Scene]:
+    """256 scenes, counts log-uniform in [1,024, 65,536]: dam breaks, sphere drops and
+    sloshing tanks."""
+    return batch_mix(256, 1024, 65536, seed, kinds=("dam_break", "sphere_drop", "sloshing"))

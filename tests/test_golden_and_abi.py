@@ -71,3 +71,14 @@ def test_create_without_device_fails_loudly():
     with pytest.raises(pcisph_amd.PcisphError) as e:
         pcisph_amd.Fluid()
     assert "NO_DEVICE" in str(e.value)
+
+
+def test_zero_radius_params_terminate(oracle_lib):
+    """A scene without a particle radius (e.g. a fresh scene of a batch) derives its
+    parameters without entering the scaling factor's lattice loop, which would not
+    advance at zero spacing; both implementations give the same bytes."""
+    import pcisph_amd
+    a = pcisph_amd.compute_params(0.0, 1000.0, 0, 0, 0.0, 0.0, 0.0, 5e-4)
+    b = oracle_lib.compute_params(0.0, 1000.0, 0, 0, 0.0, 0.0, 0.0, 5e-4)
+    assert a.raw() == b.raw()
+    assert a.density_variation_scaling_factor == 0.0 and a.bucket_count == 64

@@ -1,0 +1,153 @@
+"""GPU: batches of scenes stepped together (one launch per stage over scene tables).
+
+A scene's particles never interact with another scene's, the concatenated sort keeps
+every scene in its own sort order, and each scene's loop ends on its own test, so a
+scene stepped inside a batch must leave exactly the bits it leaves when stepped alone
+-- in every precision, with or without neighbour lists and warm start.  Together with
+the single-scene parity tests against the oracle this pins every scene of a batch to
+the oracle.
+"""
+import numpy as np
+import pytest
+
+from pcisph_amd import scenes
+from pcisph_amd.abi import Field
+from pcisph_amd.warm_start import init_params
+from tests.helpers import FLOAT_FIELDS, INT_FIELDS, run, small_scenes, snapshot
+
+pytestmark = pytest.mark.gpu
+
+STAT_KEYS = ("iterations", "converged", "max_density_variation", "max_density_error", "fluid_candidates",
+             "boundary_candidates", "warm_start", "list_passes")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import pcisph_amd
+    return pcisph_amd
+
+
+def mixed_scenes():
+    """Three scenes of different sizes, bucket counts and particle radii, with
+    different thresholds so their loops end at different iterations."""
+    a, b = small_scenes()
+    c = scenes.scaled_sphere_drop(1500, seed=5, name="sphere_drop_small")
+    b.threshold = 0.02
+    c.threshold = 0.005
+    return [a, b, c]
+
+
+def run_batch(g, batch, steps, shifts=None, **opts):
+    g.load_scenes(batch, **opts)
+    out = []
+    for k in range(steps):
+        if shifts is not None:
+            g.translate_boundaries(shifts(k))
+        out.append(g.step_scenes())
+    return out
+
+
+def compare_scene(g, k, alone, stats_batch, stats_alone, fields):
+    for sb, sa in zip(stats_batch, stats_alone):
+        for key in STAT_KEYS:
+            assert sb[k][key] == sa[key], (k, key, sb[k][key], sa[key])
+    g.select_scene(k)
+    rb, ra = snapshot(g, fields), snapshot(alone, fields)
+    for name in ra:
+        np.testing.assert_array_equal(rb[name], ra[name], err_msg=f"scene {k} {name}")
+
+
+@pytest.mark.parametrize("precision", ["ieee", "fast", "fp64"])
+def test_batch_equals_scenes_alone(gpu, precision):
+    prec = {"ieee": gpu.PRECISION_IEEE, "fast": gpu.PRECISION_FAST, "fp64": gpu.PRECISION_FP64}[precision]
+    batch = mixed_scenes()
+    print(f"case batch_equals_alone {precision} N={[s.fluid_count for s in batch]}", flush=True)
+    g = gpu.Fluid(precision=prec)
+    sb = run_batch(g, batch, 3, max_iterations=20)
+    assert len({s["iterations"] for s in sb[0]}) > 1, "thresholds should end the loops at different iterations"
+    fields = FLOAT_FIELDS + INT_FIELDS if precision != "fp64" else INT_FIELDS
+    for k, sc in enumerate(batch):
+        alone = gpu.Fluid(precision=prec)
+        sa = run(alone, sc, 3, max_iterations=20)
+        compare_scene(g, k, alone, sb, sa, fields)
+        if precision == "fp64":
+            g.select_scene(k)
+            for f in FLOAT_FIELDS:
+                np.testing.assert_array_equal(g.download_f64(f), alone.download_f64(f), err_msg=Field(f).name)
+
+
+def test_batch_without_lists_and_with_empty_scene(gpu):
+    a, b = small_scenes()
+    empty = scenes.Scene(name="empty", fluid=np.zeros(0, np.float32), boundary=np.zeros(0, np.float32),
+                         particle_radius=0.01)
+    batch = [a, empty, b]
+    print("case batch_empty_scene_no_lists", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    sb = run_batch(g, batch, 2, max_iterations=20, neighbor_skin=0.0)
+    assert all(s[1]["iterations"] == 0 and s[1]["fluid_candidates"] == 0 for s in sb)
+    for k in (0, 2):
+        alone = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+        sa = run(alone, batch[k], 2, max_iterations=20, neighbor_skin=0.0)
+        compare_scene(g, k, alone, sb, sa, FLOAT_FIELDS + INT_FIELDS)
+    # the aggregate of pcisph_step: the largest iteration count of the batch
+    agg = g.step()
+    assert agg["iterations"] >= 1
+
+
+def test_batch_warm_start_equals_scenes_alone(gpu):
+    batch = mixed_scenes()
+    params = init_params(3)
+    params[-1] = 0.5  # a positive output bias, so the network's start is a real alternative
+    print("case batch_warm_start", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    g.mlp_set_params(params)
+    g.set_warm_start(True)
+    sb = run_batch(g, batch, 3, max_iterations=20)
+    assert all(s["warm_start"] & 1 for step in sb for s in step)
+    for k, sc in enumerate(batch):
+        alone = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+        alone.mlp_set_params(params)
+        alone.set_warm_start(True)
+        sa = run(alone, sc, 3, max_iterations=20)
+        compare_scene(g, k, alone, sb, sa, FLOAT_FIELDS + INT_FIELDS)
+
+
+def test_batch_moving_walls(gpu):
+    """Sloshing tanks: pcisph_translate_boundaries moves each scene's walls by its own
+    shift in one launch; each scene equals the scene alone moved with
+    pcisph_translate_boundary."""
+    a, b = small_scenes()
+    a = scenes.sloshing_tank(0.05, 2.0, a)
+    b = scenes.sloshing_tank(0.08, 1.5, b)
+    batch = [a, b]
+    print("case batch_moving_walls", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    sb = run_batch(g, batch, 3, shifts=lambda k: [scenes.boundary_shift(sc, k) for sc in batch],
+                   max_iterations=20)
+    for k, sc in enumerate(batch):
+        alone = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+        alone.load_scene(sc, max_iterations=20)
+        sa = []
+        for step in range(3):
+            alone.translate_boundary(*scenes.boundary_shift(sc, step))
+            sa.append(alone.step())
+        compare_scene(g, k, alone, sb, sa, FLOAT_FIELDS + INT_FIELDS)
+
+
+def test_scene_count_change_keeps_scenes(gpu):
+    a, b = small_scenes()
+    print("case scene_count_change", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    g.load_scenes([a, b], max_iterations=20)
+    g.set_scene_count(3)
+    assert g.scene_count() == 3
+    for k, sc in enumerate([a, b]):
+        g.select_scene(k)
+        np.testing.assert_array_equal(g.download(Field.FLUID_POSITIONS), sc.fluid)
+        np.testing.assert_array_equal(g.download(Field.BOUNDARY_POSITIONS), sc.boundary)
+    g.select_scene(2)
+    assert g.field_size(Field.FLUID_POSITIONS) == 0
+    st = g.step_scenes()
+    assert st[2]["iterations"] == 0 and st[0]["iterations"] >= 3 and st[1]["iterations"] >= 3
+    with pytest.raises(Exception):
+        g.select_scene(3)
