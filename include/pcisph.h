@@ -96,7 +96,14 @@ typedef struct pcisph_solver_options {
                                        candidates contribute exact zeros); an iteration whose
                                        predicted positions moved more than 0.45 skin h falls
                                        back to the bucket walk.  0 = off; default 0.2 */
-    uint32_t reserved;
+    uint32_t graph_iterations;      /* > 0: the step's launches are replayed from HIP graphs --
+                                       one for everything before the correction loop and one
+                                       per `graph_iterations` loop iterations (iterations past
+                                       the end of every scene's loop return at once on the
+                                       device).  Graphs are captured on first use and again
+                                       whenever a launch argument changes; results are
+                                       identical to 0 (launch by launch).  Default 4.  Stage
+                                       profiling launches eagerly whatever this says. */
 } pcisph_solver_options;
 
 /* Per-step report (the reference only has a compiled-out timer, Fluid.cpp:319-326). */

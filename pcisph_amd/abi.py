@@ -52,7 +52,7 @@ class SolverOptions(C.Structure):
         ("block_size", C.c_uint32),
         ("sort_digit_bits", C.c_uint32),
         ("neighbor_skin", C.c_float),
-        ("reserved", C.c_uint32),
+        ("graph_iterations", C.c_uint32),
     ]
 
 
@@ -178,9 +178,10 @@ class SolverBase:
         return p
 
     def set_solver_options(self, min_iterations=3, max_iterations=7, launch_boundary_init=0,
-                           resort_boundary=0, block_size=0, sort_digit_bits=0, neighbor_skin=0.2):
+                           resort_boundary=0, block_size=0, sort_digit_bits=0, neighbor_skin=0.2,
+                           graph_iterations=4):
         o = SolverOptions(min_iterations, max_iterations, launch_boundary_init, resort_boundary,
-                          block_size, sort_digit_bits, neighbor_skin)
+                          block_size, sort_digit_bits, neighbor_skin, graph_iterations)
         self._call("set_solver_options", C.byref(o))
 
     def field_size(self, field: int) -> int:

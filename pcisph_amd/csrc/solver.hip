@@ -164,6 +164,8 @@ class StageProfiler {
         ++step_;
     }
 
+    bool on() const { return on_; }
+
     void maybe_resolve(hipStream_t s) {
         if (pending_.size() > 8192) resolve(s);
     }
@@ -335,6 +337,8 @@ class Engine final : public EngineBase {
 
     ~Engine() override {
         (void)hipStreamSynchronize(stream_);
+        drop_graphs();
+        dfree(ctl_);
         free_public();
         free_internal();
         dfree(staging_);
@@ -726,6 +730,64 @@ class Engine final : public EngineBase {
         scene_host_ = t;
     }
 
+    // Captures the launches `fn` makes on the stream into an executable graph.
+    template <typename F>
+    hipGraphExec_t capture(F&& fn) {
+        HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            fn();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamEndCapture(stream_, &g));
+        hipGraphExec_t e = nullptr;
+        const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_CHECK(rc);
+        return e;
+    }
+
+    void drop_graphs() {
+        if (prologue_graph_) (void)hipGraphExecDestroy(prologue_graph_);
+        prologue_graph_ = nullptr;
+        for (hipGraphExec_t e : loop_graphs_)
+            if (e) (void)hipGraphExecDestroy(e);
+        loop_graphs_.clear();
+        graph_key_.clear();
+    }
+
+    // Every value a recorded launch bakes in; graphs are re-captured when it changes.
+    std::vector<uint64_t> launch_key(const Launch& L, bool fast, bool warm, bool features, uint32_t digit,
+                                     uint32_t key_bits, uint32_t maxit, uint32_t chunk, const float* mlp) const {
+        std::vector<uint64_t> k;
+        auto put = [&](uint64_t v) { k.push_back(v); };
+        auto ptr = [&](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
+        put(L.block), put(fast), put(warm), put(features), put(digit), put(key_bits), put(maxit), put(chunk);
+        put(A_.n), put(A_.nb), put(A_.B);
+        for (const void* p : {(const void*)A_.pos_in, (const void*)A_.vel_in, (const void*)A_.pos,
+                              (const void*)A_.vel, (const void*)A_.xr, (const void*)A_.nrm, (const void*)A_.fo,
+                              (const void*)A_.fp, (const void*)A_.xp, (const void*)A_.xf, (const void*)A_.p,
+                              (const void*)A_.dv, (const void*)A_.counts, (const void*)A_.keys[0],
+                              (const void*)A_.keys[1], (const void*)A_.vals[0], (const void*)A_.vals[1],
+                              (const void*)A_.cells, (const void*)A_.bpos, (const void*)A_.bxf, (const void*)A_.bp,
+                              (const void*)A_.binit, (const void*)A_.bcells, (const void*)sort_scratch_,
+                              (const void*)feat_, (const void*)mlp, (const void*)ctl_, (const void*)dev_host_,
+                              (const void*)dev_results_, (const void*)D_.xp0, (const void*)D_.xf0,
+                              (const void*)D_.p0, (const void*)D_.dv0, (const void*)D_.bp0, (const void*)D_.bxf0})
+            ptr(p);
+        const Batch<R>& B = A_.batch;
+        ptr(B.scenes), ptr(B.blocks), ptr(B.states);
+        put(B.scene_count), put(B.fluid_blocks), put(B.boundary_blocks), put(B.n_total), put(B.nb_total);
+        const NeighborLists& NL = A_.NL;
+        ptr(NL.f), ptr(NL.b), ptr(NL.bf), ptr(NL.fcount), ptr(NL.bcount), ptr(NL.bfcount);
+        put(NL.cap_f), put(NL.cap_b), put(NL.cap_bf), put(NL.on);
+        return k;
+    }
+
     void step(const std::vector<SceneInput>& scenes, const pcisph_solver_options& o, bool sort_boundary, bool fast,
               const WarmArgs& w, std::vector<StepResult>& out, uint64_t* samples) override {
         HIP_CHECK(hipSetDevice(device_));
@@ -737,12 +799,27 @@ class Engine final : public EngineBase {
         const unsigned int tag = ++tag_ ? tag_ : ++tag_;
         const bool warm = w.params != nullptr;
         const bool features = warm || w.collect_stride;
+        const uint32_t maxit = o.max_iterations, minit = std::max(o.min_iterations, 1u);
         prepare_blocks(L.block);
         setup_lists(o.neighbor_skin);
         prepare_scenes(scenes, o);
-        A_.batch = Batch<R>{scene_table_, block_table_, states_, S, fluid_blocks_, boundary_blocks_, A_.n, A_.nb};
+        A_.batch = Batch<R>{scene_table_, block_table_, states_, S, fluid_blocks_, boundary_blocks_, A_.n, A_.nb, 0};
         if (features) ensure_warm_buffers(warm);
+        if (!ctl_) dalloc(ctl_, 64, stream_);
+        // graphs record launches; the stage profiler times them one by one
+        const uint32_t chunk = prof_.on() ? 0u : o.graph_iterations;
+        if (chunk) {
+            std::vector<uint64_t> key = launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, w.params);
+            if (key != graph_key_) {
+                drop_graphs();
+                graph_key_ = key;
+            }
+        } else if (!graph_key_.empty()) {
+            drop_graphs();
+        }
+        // this step's loop records and tag (outside any graph: the tag changes per step)
         fill_u64(L, (unsigned long long*)states_, sizeof(LoopState) / 8 * (size_t)S, 0ull);
+        fill_u32(L, ctl_, 1, tag);
         if (sort_boundary && A_.nb) {
             prof_.timed(PCISPH_STAGE_BOUNDARY_SORT, stream_, -1, [&] {
                 fill_u32(L, (uint32_t*)A_.bcells, 2 * (size_t)A_.B, 0u);
@@ -756,47 +833,75 @@ class Engine final : public EngineBase {
                 if (o.launch_boundary_init) launch(fast).boundary_init(L, A_);
             });
         }
-        prof_.timed(PCISPH_STAGE_HASH, stream_, -1, [&] {
-            fill_u32(L, (uint32_t*)A_.cells, 2 * (size_t)A_.B, 0u);
-            DataLaunch<R>::hash(L, A_.batch, false, A_.pos_in, A_.keys[0], A_.vals[0]);
-        });
-        prof_.timed(PCISPH_STAGE_SORT, stream_, -1, [&] {
-            fluid_sorted_ = radix_sort(L, A_.keys, A_.vals, A_.n, key_bits, digit, sort_scratch_);
-        });
-        if (fluid_sorted_ < 0) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "too many particles for one sort");
-        prof_.timed(PCISPH_STAGE_REORDER, stream_, -1, [&] {
-            DataLaunch<R>::reorder(L, A_.n, A_.keys[fluid_sorted_], A_.vals[fluid_sorted_], A_.pos_in,
-                                   A_.vel_in, A_.pos, A_.vel, A_.cells);
-        });
         const LaunchTable ops = launch(fast);
-        prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
-        prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, A_); });
-        FeatureArgs F{features ? feat_ : nullptr, A_.counts};
-        prof_.timed(PCISPH_STAGE_FORCE_INIT, stream_, -1, [&] { ops.force_init(L, A_, F); });
-        if (warm) {
-            // network start: p from the features, its pressure force, and a prediction
-            // with it; the plain prediction is kept for the dual first iteration
-            prof_.timed(PCISPH_STAGE_WARM_START, stream_, -1, [&] {
-                mlp_warm_start<R>(ML, w.params, feat_, A_.n, A_.p);
-                DataLaunch<R>::warm_apply(L, A_);
-                ops.pressure_force(L, A_, 0, dev_host_, dev_results_, tag, 1, D_.xp0);
+        // everything before the correction loop
+        auto prologue = [&] {
+            prof_.timed(PCISPH_STAGE_HASH, stream_, -1, [&] {
+                fill_u32(L, (uint32_t*)A_.cells, 2 * (size_t)A_.B, 0u);
+                DataLaunch<R>::hash(L, A_.batch, false, A_.pos_in, A_.keys[0], A_.vals[0]);
             });
+            prof_.timed(PCISPH_STAGE_SORT, stream_, -1, [&] {
+                fluid_sorted_ = radix_sort(L, A_.keys, A_.vals, A_.n, key_bits, digit, sort_scratch_);
+            });
+            if (fluid_sorted_ < 0) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "too many particles for one sort");
+            prof_.timed(PCISPH_STAGE_REORDER, stream_, -1, [&] {
+                DataLaunch<R>::reorder(L, A_.n, A_.keys[fluid_sorted_], A_.vals[fluid_sorted_], A_.pos_in,
+                                       A_.vel_in, A_.pos, A_.vel, A_.cells);
+            });
+            prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
+            prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, A_); });
+            FeatureArgs F{features ? feat_ : nullptr, A_.counts};
+            prof_.timed(PCISPH_STAGE_FORCE_INIT, stream_, -1, [&] { ops.force_init(L, A_, F); });
+            if (warm) {
+                // network start: p from the features, its pressure force, and a prediction
+                // with it; the plain prediction is kept for the dual first iteration
+                prof_.timed(PCISPH_STAGE_WARM_START, stream_, -1, [&] {
+                    mlp_warm_start<R>(ML, w.params, feat_, A_.n, A_.p);
+                    DataLaunch<R>::warm_apply(L, A_);
+                    ops.pressure_force(L, A_, 0, dev_host_, dev_results_, ctl_, 1, D_.xp0);
+                });
+            }
+        };
+        auto iteration = [&](uint32_t it) {
+            prof_.timed(PCISPH_STAGE_PRESSURE, stream_, (int)it, [&] {
+                if (warm && it == 0) ops.pressure_dual(L, A_, D_);
+                else ops.pressure(L, A_, it);
+            });
+            prof_.timed(PCISPH_STAGE_PRESSURE_FORCE, stream_, (int)it, [&] {
+                ops.pressure_force(L, A_, it, dev_host_, dev_results_, ctl_, 0, nullptr);
+            });
+        };
+        if (chunk) {
+            if (!prologue_graph_) {
+                prologue_graph_ = capture(prologue);
+                prologue_sorted_ = fluid_sorted_;
+            }
+            fluid_sorted_ = prologue_sorted_;
+            HIP_CHECK(hipGraphLaunch(prologue_graph_, stream_));
+        } else {
+            prologue();
         }
         HIP_CHECK(hipGetLastError());
 
-        const uint32_t maxit = o.max_iterations, minit = std::max(o.min_iterations, 1u);
+        // Iterations are enqueued `step` at a time (one graph, or one iteration), keeping
+        // at least one iteration queued beyond the one the host waits for.
         const uint32_t first = std::min(minit - 1, maxit - 1);
         const uint32_t lookahead = 1;
+        const uint32_t step_iters = chunk ? chunk : 1u;
         uint32_t enq = 0;
         auto enqueue = [&]() {
-            prof_.timed(PCISPH_STAGE_PRESSURE, stream_, (int)enq, [&] {
-                if (warm && enq == 0) ops.pressure_dual(L, A_, D_);
-                else ops.pressure(L, A_, enq);
-            });
-            prof_.timed(PCISPH_STAGE_PRESSURE_FORCE, stream_, (int)enq, [&] {
-                ops.pressure_force(L, A_, enq, dev_host_, dev_results_, tag, 0, nullptr);
-            });
-            ++enq;
+            if (chunk) {
+                const uint32_t c = enq / chunk;
+                if (loop_graphs_.size() <= c) loop_graphs_.resize(c + 1, nullptr);
+                if (!loop_graphs_[c])
+                    loop_graphs_[c] = capture([&] {
+                        for (uint32_t it = enq; it < std::min(enq + chunk, maxit); ++it) iteration(it);
+                    });
+                HIP_CHECK(hipGraphLaunch(loop_graphs_[c], stream_));
+            } else {
+                iteration(enq);
+            }
+            enq = std::min(enq + step_iters, maxit);
         };
         while (enq < maxit && enq <= first + lookahead) enqueue();
         HIP_CHECK(hipGetLastError());
@@ -806,7 +911,7 @@ class Engine final : public EngineBase {
             if (host_->running[k] == 0) break;
             if (k + 1 >= maxit) throw Error(PCISPH_ERR_INTERNAL, "scenes still running after the last iteration");
             ++k;
-            if (enq < maxit) enqueue();
+            if (enq < maxit && enq <= k + lookahead) enqueue();
         }
         HIP_CHECK(hipGetLastError());
         uint64_t collected = 0;
@@ -899,7 +1004,7 @@ class Engine final : public EngineBase {
         void (*force_init)(const Launch&, const DeviceArrays<R>&, const FeatureArgs&);
         void (*pressure)(const Launch&, const DeviceArrays<R>&, uint32_t);
         void (*pressure_force)(const Launch&, const DeviceArrays<R>&, uint32_t, HostMirror*, SceneResult*,
-                               unsigned int, int, V4<R>*);
+                               const unsigned int*, int, V4<R>*);
         void (*boundary_init)(const Launch&, const DeviceArrays<R>&);
         void (*pressure_dual)(const Launch&, const DeviceArrays<R>&, const DualArrays<R>&);
     };
@@ -927,6 +1032,11 @@ class Engine final : public EngineBase {
     float* feat_ = nullptr;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
     float list_skin_ = 0.f;
+    uint32_t* ctl_ = nullptr;  // device word: the tag of the step in flight
+    hipGraphExec_t prologue_graph_ = nullptr;
+    int prologue_sorted_ = 0;
+    std::vector<hipGraphExec_t> loop_graphs_;  // [c]: iterations c*chunk .. (c+1)*chunk-1
+    std::vector<uint64_t> graph_key_;
     float* shift_host_ = nullptr;  // pinned staging of pcisph_translate_boundaries
     float* shift_dev_ = nullptr;
     size_t shift_cap_ = 0;
@@ -983,7 +1093,7 @@ struct pcisph_solver {
     bool own_stream = false;
     std::vector<Scene> scenes = std::vector<Scene>(1);
     uint32_t current = 0;  // the scene that setters, uploads and downloads address
-    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 0};
+    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 4};
     int precision = kPrecisionFast;
     std::unique_ptr<EngineBase> eng;
     std::string err;

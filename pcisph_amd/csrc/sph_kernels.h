@@ -142,6 +142,7 @@ struct Batch {
     uint32_t scene_count;
     uint32_t fluid_blocks, boundary_blocks;
     uint32_t n_total, nb_total;   // list strides
+    uint32_t pad;
 };
 
 template <typename R>
@@ -773,10 +774,11 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
 // (the pressure pass that produced the results has completed): every scene that ran
 // the iteration writes its results, the number of scenes that go on is stored, and
 // then the step tag, so the host decides how far to enqueue without waiting on the
-// stream.
+// stream.  The tag is read from device memory (written at the start of each step),
+// so a launch recorded in a graph publishes the tag of the step that replays it.
 template <typename R>
 __device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* host, SceneResult* res,
-                                  unsigned int tag) {
+                                  const unsigned int* tag) {
     __shared__ unsigned int s_run[16];
     unsigned int running = 0;
     for (uint32_t s = threadIdx.x; s < B.scene_count; s += blockDim.x) {
@@ -807,7 +809,7 @@ __device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* 
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += s_run[w];
         host->running[iter] = total;
         __threadfence_system();
-        __hip_atomic_store(&host->tag[iter], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&host->tag[iter], *tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -821,8 +823,9 @@ __global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restr
                                  const V4<R>* __restrict__ xf, const V4<R>* __restrict__ fo,
                                  const V4<R>* __restrict__ vel, V4<R>* __restrict__ fp, V4<R>* __restrict__ xp,
                                  V4<R>* __restrict__ pos_out, V4<R>* __restrict__ vel_out,
-                                 const R* __restrict__ p, HostMirror* host, SceneResult* res, unsigned int tag,
-                                 int warm_pass, V4<R>* __restrict__ xp_save, NeighborLists NL) {
+                                 const R* __restrict__ p, HostMirror* host, SceneResult* res,
+                                 const unsigned int* tag, int warm_pass, V4<R>* __restrict__ xp_save,
+                                 NeighborLists NL) {
     using O = Ops<R, FAST>;
     // warm_pass: the pressure force of the network's initial pressures, before the
     // first prediction -- not an iteration of the loop.

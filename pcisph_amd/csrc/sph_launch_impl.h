@@ -44,7 +44,8 @@ void SphLaunch<R, FAST>::pressure(const Launch& L, const DeviceArrays<R>& A, uin
 
 template <typename R, bool FAST>
 void SphLaunch<R, FAST>::pressure_force(const Launch& L, const DeviceArrays<R>& A, uint32_t iter, HostMirror* host,
-                                        SceneResult* res, unsigned int tag, int warm_pass, V4<R>* xp_save) {
+                                        SceneResult* res, const unsigned int* tag, int warm_pass,
+                                        V4<R>* xp_save) {
     const Batch<R>& B = A.batch;
     if (!B.fluid_blocks) return;
     hipLaunchKernelGGL((k_pressure_force<R, FAST>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B, iter,
