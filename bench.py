@@ -63,6 +63,68 @@ def cpu_baseline(seconds: float, max_steps: int) -> dict:
                       f"{threads} threads, mean {np.mean(its):.2f} iterations/step"}
 
 
+def roofline_of(times: dict, n: int, cf: float, cb: float) -> tuple[dict, dict]:
+    """Per-stage device times and the roofline figure of the dominant neighbour kernel."""
+    stage_ms = {STAGES[i]: {"ms_total": round(times["ms"][i], 3), "launches": times["launches"][i],
+                            "us_per_launch": round(1e3 * times["ms"][i] / times["launches"][i], 2)
+                            if times["launches"][i] else None}
+                for i in range(len(STAGES))}
+    dom = max(STAGE_BYTES, key=lambda k: times["ms"][STAGES.index(k)])
+    i = STAGES.index(dom)
+    avg_s = times["ms"][i] / times["launches"][i] / 1e3
+    bytes_per_launch = STAGE_BYTES[dom](cf, cb) * n
+    achieved = bytes_per_launch / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
+    return stage_ms, roofline
+
+
+def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None) -> dict:
+    """`warmup` untimed then `steps` timed steps (every scene of a batch), HIP-event
+    stage times over the timed region when `profile`."""
+    shift = None
+    if batch is not None and any(sc.meta.get("slosh") for sc in batch):
+        from pcisph_amd import scenes
+        shift = lambda k: [scenes.boundary_shift(sc, k) for sc in batch]  # noqa: E731
+
+    def go(count, start):
+        out = []
+        for k in range(count):
+            if shift:
+                fluid.translate_boundaries(shift(start + k))
+            out.append(fluid.step_scenes() if batch is not None else [fluid.step()])
+        return out
+
+    go(warmup, 0)
+    fluid.sync()
+    if profile:
+        fluid.profile_enable(True)
+    t0 = time.perf_counter()
+    stats = go(steps, warmup)
+    fluid.sync()
+    elapsed = time.perf_counter() - t0
+    times = fluid.profile_read() if profile else None
+    if profile:
+        fluid.profile_enable(False)
+    its = np.array([[q["iterations"] for q in st] for st in stats], dtype=np.float64)
+    err = np.array([[q["max_density_error"] for q in st] for st in stats], dtype=np.float64)
+    conv = np.array([[q["converged"] for q in st] for st in stats])
+    live = its > 0
+    cf = float(np.sum([[q["fluid_candidates"] for q in st] for st in stats])) / (n * steps)
+    cb = float(np.sum([[q["boundary_candidates"] for q in st] for st in stats])) / (n * steps)
+    out = {"value": round(n * steps / elapsed, 1), "ms_per_step": round(1e3 * elapsed / steps, 4),
+           "steps_per_s": round(steps / elapsed, 2), "mean_iterations": round(float(its[live].mean()), 3),
+           "max_iterations_seen": int(its.max()), "converged_fraction": round(float(conv[live].mean()), 4),
+           "max_density_error_max": round(float(err.max()), 5),
+           "mean_fluid_candidates": round(cf, 2), "mean_boundary_candidates": round(cb, 2)}
+    if batch is not None:
+        out["mean_iterations_slowest_scene"] = round(float(its.max(axis=1).mean()), 3)
+    if times is not None:
+        out["stage_times"], out["roofline"] = roofline_of(times, n, cf, cb)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +135,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
+    ap.add_argument("--forms", default="plain,warm_start,batch_64",
+                    help="comma list of plain, warm_start (trained network), batch_64 (64-scene batch)")
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "pcisph_amd", "data", "warm_start.npy"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -82,66 +148,62 @@ def main() -> int:
     import pcisph_amd
     from pcisph_amd import scenes
 
-    scene = getattr(scenes, args.workload)()
+    forms = [f for f in args.forms.split(",") if f]
     prec = {"fast": pcisph_amd.PRECISION_FAST, "ieee": pcisph_amd.PRECISION_IEEE,
             "fp64": pcisph_amd.PRECISION_FP64}[args.precision]
-    fluid = pcisph_amd.Fluid(device=0, precision=prec)
-    fluid.load_scene(scene)
-    print(f"bench: {scene.name} N={scene.fluid_count} Nb={scene.boundary_count} "
-          f"warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
-    if args.warmup:
-        fluid.run(args.warmup, collect=False)
-    fluid.sync()
-    if not args.no_profile:
-        fluid.profile_enable(True)
-    t0 = time.perf_counter()
-    stats = fluid.run(args.steps, collect=True)
-    fluid.sync()
-    elapsed = time.perf_counter() - t0
-    times = fluid.profile_read() if not args.no_profile else None
+    scene = getattr(scenes, args.workload)()
     n = scene.fluid_count
-    its = np.array([s["iterations"] for s in stats], dtype=np.float64)
-    err = np.array([s["max_density_error"] for s in stats], dtype=np.float64)
-    conv = np.array([s["converged"] for s in stats])
-    cf = float(np.mean([s["fluid_candidates"] for s in stats])) / n
-    cb = float(np.mean([s["boundary_candidates"] for s in stats])) / n
-    value = n * args.steps / elapsed
+    results = {}
+    for form in forms:
+        if form == "batch_64":
+            batch = scenes.batch_64()
+            total = sum(sc.fluid_count for sc in batch)
+            fluid = pcisph_amd.Fluid(device=0, precision=prec)
+            fluid.load_scenes(batch)
+            print(f"bench: batch_64 {len(batch)} scenes N={total} steps={args.batch_steps}", file=sys.stderr,
+                  flush=True)
+            r = measure(fluid, args.batch_steps, args.warmup, total, not args.no_profile, batch=batch)
+            # scenes whose loop has ended skip the remaining iterations, so the per-particle
+            # byte model of a launch does not hold for a batch: stage times only
+            r.pop("roofline", None)
+            r.update({"scenes": len(batch), "fluid_particles": total,
+                      "boundary_particles": sum(sc.boundary_count for sc in batch)})
+        else:
+            fluid = pcisph_amd.Fluid(device=0, precision=prec)
+            fluid.load_scene(scene)
+            if form == "warm_start":
+                if not os.path.exists(args.weights):
+                    print(f"bench: no trained network at {args.weights}; warm_start skipped", file=sys.stderr)
+                    continue
+                fluid.mlp_set_params(np.load(args.weights))
+                fluid.set_warm_start(True)
+            print(f"bench: {form} {scene.name} N={n} Nb={scene.boundary_count} warmup={args.warmup} "
+                  f"steps={args.steps}", file=sys.stderr, flush=True)
+            r = measure(fluid, args.steps, args.warmup, n, not args.no_profile)
+        fluid.close()
+        results[form] = r
+        print(f"bench: {form} {json.dumps({k: v for k, v in r.items() if k != 'stage_times'})}",
+              file=sys.stderr, flush=True)
 
-    roofline = None
-    stage_ms = None
-    if times is not None:
-        stage_ms = {STAGES[i]: {"ms_total": round(times["ms"][i], 3), "launches": times["launches"][i],
-                                "us_per_launch": round(1e3 * times["ms"][i] / times["launches"][i], 2)
-                                if times["launches"][i] else None}
-                    for i in range(len(STAGES))}
-        dom = max(STAGE_BYTES, key=lambda k: times["ms"][STAGES.index(k)])
-        i = STAGES.index(dom)
-        avg_s = times["ms"][i] / times["launches"][i] / 1e3
-        bytes_per_launch = STAGE_BYTES[dom](cf, cb) * n
-        achieved = bytes_per_launch / avg_s / 1e9
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "bytes_per_launch": int(bytes_per_launch),
-                    "avg_launch_us": round(avg_s * 1e6, 2)}
-
+    head = "plain" if "plain" in results else next(iter(results))
+    main_r = results[head]
     cpu = None
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_seconds, 200)
 
     line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": METRIC, "value": main_r["value"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps if head != "batch_64" else args.batch_steps, "warmup": args.warmup,
+        "ms_per_step": main_r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32", "data": "synthetic",
-        "config": {"workload": scene.name, "fluid_particles": n, "boundary_particles": scene.boundary_count,
-                   "precision": args.precision, "mean_iterations": round(float(its.mean()), 3),
-                   "max_iterations_seen": int(its.max()), "steps_converged": int(conv.sum()),
-                   "max_density_error_max": round(float(err.max()), 5),
-                   "steps_per_s": round(args.steps / elapsed, 2),
-                   "mean_fluid_candidates": round(cf, 2), "mean_boundary_candidates": round(cb, 2),
-                   "stage_times": stage_ms},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "config": {"workload": scene.name if head != "batch_64" else "batch_64", "form": head,
+                   "fluid_particles": n, "boundary_particles": scene.boundary_count,
+                   "precision": args.precision, "density_error_bound": 0.01,
+                   "iterations_min_max": [3, 50],
+                   **{k: v for k, v in main_r.items() if k not in ("value", "ms_per_step", "roofline")}},
+        "roofline": main_r.get("roofline"), "cpu_baseline": cpu,
+        "forms": {k: {q: v for q, v in r.items() if q != "stage_times"} for k, r in results.items()},
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -296,6 +296,14 @@ int pcisph_translate_boundaries(pcisph_solver* s, const float* shifts, size_t co
  * tuner, exec_clogs_tune.bat). */
 int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen);
 
+/* Device-side checkpoint of every scene's state at a step boundary: fluid positions and
+ * velocities (with the previous step's pressures the warm start reads) and boundary
+ * positions.  pcisph_restore puts it back (same counts), so runs can be repeated from
+ * one state; pending setter changes are applied before the snapshot is taken.  No
+ * counterpart in the reference. */
+int pcisph_snapshot(pcisph_solver* s);
+int pcisph_restore(pcisph_solver* s);
+
 /* Derived parameters for a parameter set without a device (host only). */
 int pcisph_compute_params(float particle_radius, float rest_density, uint32_t boundary_count,
                           uint32_t fluid_count, float viscosity, float surface_tension,

@@ -38,7 +38,7 @@ EXPORTS = (
     "pcisph_samples_download", "pcisph_samples_upload", "pcisph_mlp_train", "pcisph_set_warm_start",
     "pcisph_autotune", "pcisph_translate_boundary",
     "pcisph_set_scene_count", "pcisph_get_scene_count", "pcisph_select_scene", "pcisph_step_scenes",
-    "pcisph_translate_boundaries",
+    "pcisph_translate_boundaries", "pcisph_snapshot", "pcisph_restore",
 )
 
 MLP_INPUTS = 8
@@ -243,6 +243,14 @@ class Fluid(SolverBase):
         """Move every scene's walls at once: shifts[k] = (dx, dy, dz) of scene k."""
         a = np.ascontiguousarray(shifts, np.float32).ravel()
         self._call("translate_boundaries", a.ctypes.data_as(C.c_void_p), C.c_size_t(a.size))
+
+    def snapshot(self):
+        """Keep the state of every scene on the device (see restore)."""
+        self._call("snapshot")
+
+    def restore(self):
+        """Back to the last snapshot, bit for bit."""
+        self._call("restore")
 
     def set_warm_start(self, on: bool = True):
         self._call("set_warm_start", C.c_int(1 if on else 0))

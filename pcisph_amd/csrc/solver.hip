@@ -1717,6 +1717,17 @@ int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_option
     });
 }
 
+int pcisph_snapshot(pcisph_solver* s) {
+    return guarded(s, [&] {
+        apply_param_changes(s);
+        s->eng->snapshot();
+    });
+}
+
+int pcisph_restore(pcisph_solver* s) {
+    return guarded(s, [&] { s->eng->restore(); });
+}
+
 int pcisph_profile_enable(pcisph_solver* s, int enable) {
     return guarded(s, [&] { s->eng->profile_enable(enable != 0); });
 }
