@@ -93,9 +93,12 @@ typedef struct pcisph_solver_options {
                                        the reference's summation order, every candidate within
                                        (1 + skin) h; later passes walk those lists instead of
                                        the 27 buckets.  Results are identical (the skipped
-                                       candidates contribute exact zeros); an iteration whose
-                                       predicted positions moved more than 0.45 skin h falls
-                                       back to the bucket walk.  0 = off; default 0.2 */
+                                       candidates contribute exact zeros).  In the pressure
+                                       pass, where distances are taken at predicted positions,
+                                       a wavefront walks lists only if for each of its
+                                       particles its displacement plus the largest in its 27
+                                       buckets is at most 0.9 skin h, else its buckets.
+                                       0 = off; default 0.2 */
     uint32_t graph_iterations;      /* > 0: the step's launches are replayed from HIP graphs --
                                        one for everything before the correction loop and one
                                        per `graph_iterations` loop iterations (iterations past
@@ -121,8 +124,9 @@ typedef struct pcisph_step_stats {
                                        step-start density was >= rho0 (free-surface particles
                                        sit below rho0 regardless), last iteration: the side of
                                        the residual the loop does not bound */
-    uint32_t list_passes;           /* pressure passes of this step that walked the neighbour
-                                       lists (the others walked the 27 buckets) */
+    uint32_t list_waves;            /* wavefronts (64 particles) of this step's pressure passes
+                                       that walked neighbour lists; the others walked their
+                                       27 buckets (sum over the step's iterations) */
 } pcisph_step_stats;
 
 typedef struct pcisph_solver pcisph_solver;

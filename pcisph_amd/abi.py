@@ -67,7 +67,7 @@ class StepStats(C.Structure):
         ("fluid_candidates", C.c_uint64),
         ("boundary_candidates", C.c_uint64),
         ("max_density_expansion", C.c_float),
-        ("list_passes", C.c_uint32),
+        ("list_waves", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -228,7 +228,7 @@ struct StepResult {
     double max_dv = 0, max_expansion = 0;
     uint64_t fluid_candidates = 0, boundary_candidates = 0;
     bool warm = false, warm_rejected = false;
-    uint32_t list_passes = 0;
+    uint32_t list_waves = 0;
 };
 
 // What a step does for the warm-start network: use it (params != null) and/or
@@ -350,6 +350,8 @@ class Engine final : public EngineBase {
         dfree(A_.NL.fcount);
         dfree(A_.NL.bcount);
         dfree(A_.NL.bfcount);
+        dfree(A_.NL.disp[0]);
+        dfree(A_.NL.disp[1]);
         dfree(D_.xp0);
         dfree(D_.xf0);
         dfree(D_.p0);
@@ -716,7 +718,7 @@ class Engine final : public EngineBase {
                                   (3.14159265358979323846 * std::pow((double)P.h, 6)) * (double)P.dt * 100.0);
             const float h = (float)P.h, radius = (1.f + list_skin_) * h;
             P.list_r2 = list_skin_ > 0.f ? radius * radius : 0.f;
-            P.disp_limit = 0.45f * list_skin_ * h;
+            P.disp_limit = 0.9f * list_skin_ * h;
             t[s] = P;
         }
         if (t.size() == scene_host_.size() &&
@@ -784,7 +786,8 @@ class Engine final : public EngineBase {
         put(B.scene_count), put(B.fluid_blocks), put(B.boundary_blocks), put(B.n_total), put(B.nb_total);
         const NeighborLists& NL = A_.NL;
         ptr(NL.f), ptr(NL.b), ptr(NL.bf), ptr(NL.fcount), ptr(NL.bcount), ptr(NL.bfcount);
-        put(NL.cap_f), put(NL.cap_b), put(NL.cap_bf), put(NL.on);
+        ptr(NL.disp[0]), ptr(NL.disp[1]), ptr(D_.disp_next);
+        put(NL.cap_f), put(NL.cap_b), put(NL.cap_bf), put(NL.on), put(NL.buckets), put(D_.buckets);
         return k;
     }
 
@@ -805,6 +808,8 @@ class Engine final : public EngineBase {
         prepare_scenes(scenes, o);
         A_.batch = Batch<R>{scene_table_, block_table_, states_, S, fluid_blocks_, boundary_blocks_, A_.n, A_.nb, 0};
         if (features) ensure_warm_buffers(warm);
+        D_.disp_next = A_.NL.on ? A_.NL.disp[1] : nullptr;
+        D_.buckets = A_.NL.buckets;
         if (!ctl_) dalloc(ctl_, 64, stream_);
         // graphs record launches; the stage profiler times them one by one
         const uint32_t chunk = prof_.on() ? 0u : o.graph_iterations;
@@ -948,7 +953,7 @@ class Engine final : public EngineBase {
             r.boundary_candidates = q.candidates[1];
             r.warm = warm;
             r.warm_rejected = warm && q.warm_rejected;
-            r.list_passes = q.list_passes;
+            r.list_waves = q.list_waves;
         }
     }
 
@@ -976,6 +981,12 @@ class Engine final : public EngineBase {
             list_n_ = A_.n;
             list_nb_ = A_.nb;
         }
+        if (list_buckets_ != A_.B || !NL.disp[0]) {
+            dalloc(NL.disp[0], std::max(A_.B, 1u), stream_);
+            dalloc(NL.disp[1], std::max(A_.B, 1u), stream_);
+            list_buckets_ = A_.B;
+        }
+        NL.buckets = A_.B;
         NL.on = 1;
     }
 
@@ -1032,6 +1043,7 @@ class Engine final : public EngineBase {
     float* feat_ = nullptr;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
     float list_skin_ = 0.f;
+    uint32_t list_buckets_ = 0;
     uint32_t* ctl_ = nullptr;  // device word: the tag of the step in flight
     hipGraphExec_t prologue_graph_ = nullptr;
     int prologue_sorted_ = 0;
@@ -1367,7 +1379,7 @@ static pcisph_step_stats to_stats(const pcisph_solver* s, const StepResult& r, c
     if (!p.fluid_count) return st;
     st.warm_start = (r.warm ? 1u : 0u) | (r.warm_rejected ? 2u : 0u);
     st.max_density_expansion = (float)r.max_expansion;
-    st.list_passes = r.list_passes;
+    st.list_waves = r.list_waves;
     st.iterations = r.iterations;
     st.converged = r.converged;
     st.max_density_variation = (float)r.max_dv;
@@ -1397,7 +1409,7 @@ static pcisph_step_stats combine(const std::vector<pcisph_step_stats>& v) {
         a.warm_start |= st.warm_start;
         a.fluid_candidates += st.fluid_candidates;
         a.boundary_candidates += st.boundary_candidates;
-        a.list_passes = std::max(a.list_passes, st.list_passes);
+        a.list_waves = std::max(a.list_waves, st.list_waves);
     }
     if (!any) a.converged = 0;
     return a;

@@ -61,7 +61,7 @@ struct SphParams {
     float p_ref;                     // delta * rho0 / 100: warm-start pressure unit
     float div_scale;                 // m * 45 / (pi h^6) * dt * 100: warm-start divergence unit
     float list_r2;                   // neighbour-list radius squared, ((1 + skin) h)^2
-    float disp_limit;                // largest max |xp - x| with which a pressure pass may use the lists
+    float disp_limit;                // 0.9 skin h: the largest d_i + D_i with which particle i may use its lists
 };
 
 // One workgroup's share of a launch: particles [begin, end) of one scene.
@@ -78,7 +78,6 @@ struct LoopState {
     unsigned long long max_dv[kMaxIterationsCap];
     unsigned long long max_exp[kMaxIterationsCap];  // max(0, rho0 - rho*) over particles that began the
                                                     // step at or above rho0: the other side of the residual
-    unsigned int max_disp[kMaxIterationsCap];        // float bits: max |xp - x| entering iteration k
     unsigned long long fluid_candidates;
     unsigned long long boundary_candidates;
     // iteration 0 of a warm-started step evaluates both starts: [0] warm, [1] plain
@@ -86,7 +85,7 @@ struct LoopState {
     unsigned long long dual_exp[2];
     unsigned int nonfinite;
     unsigned int warm_rejected;  // 1: the plain start had the smaller residual
-    unsigned int list_passes;    // pressure passes that walked the neighbour lists
+    unsigned int list_waves;     // wavefronts of the pressure passes that walked neighbour lists
     unsigned int pad;
 };
 
@@ -100,6 +99,9 @@ struct DualArrays {
     R* dv0;
     R* bp0;
     V4<R>* bxf0;
+    uint32_t* disp_next;  // the neighbour lists' displacement table of iteration 1 (or null)
+    uint32_t buckets;
+    uint32_t pad;
 };
 
 // Host-visible record of the correction loop (pinned, mapped memory), written by the
@@ -118,7 +120,7 @@ struct SceneResult {
     unsigned long long candidates[2];
     unsigned int iterations;
     unsigned int warm_rejected;
-    unsigned int list_passes;
+    unsigned int list_waves;
     unsigned int pad;
 };
 
@@ -280,6 +282,15 @@ __device__ __forceinline__ void for_each_candidate(const uint2* __restrict__ cel
 // neighbours than the capacity gets kListOverflow and walks its buckets.
 constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
 
+//
+// The pressure pass measures distances at predicted positions x^ but looks up the
+// 27 buckets of x (PCISPH/data/kernels/pcisph.cl:238,246).  A candidate j of those
+// buckets with |x^_i - x^_j| <= h has |x_i - x_j| <= h + d_i + d_j (d = |x^ - x|),
+// so it is in i's list whenever d_i + d_j <= skin h.  disp[k & 1] holds, per bucket,
+// the largest d of its fluid particles for iteration k's predicted positions (float
+// bits; written by the pass that predicts them, cleared one pass ahead), and
+// particle i walks its list in iteration k when d_i + D_i <= 0.9 skin h, D_i the
+// largest entry of its 27 buckets (a boundary particle, d = 0, when D_i <= 0.9 skin h).
 struct NeighborLists {
     uint32_t* f = nullptr;        // fluid neighbours of fluid particles    [cap_f][n]
     uint32_t* b = nullptr;        // boundary neighbours of fluid particles [cap_b][n]
@@ -287,9 +298,44 @@ struct NeighborLists {
     uint32_t* fcount = nullptr;   // [n]
     uint32_t* bcount = nullptr;   // [n]
     uint32_t* bfcount = nullptr;  // [nb]
+    uint32_t* disp[2] = {nullptr, nullptr};  // per-bucket max displacement, by iteration parity [B]
     uint32_t cap_f = 0, cap_b = 0, cap_bf = 0;
-    uint32_t on = 0;  // 0: no lists this step (every pass walks its buckets)
+    uint32_t on = 0;       // 0: no lists this step (every pass walks its buckets)
+    uint32_t buckets = 0;  // entries of each disp table
+    uint32_t pad = 0;
 };
+
+// Clears a displacement table: the whole grid shares the work.
+__device__ __forceinline__ void clear_table(uint32_t* t, uint32_t count) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += stride) t[q] = 0u;
+}
+
+// Records each particle's displacement in its bucket's entry: a segmented maximum
+// over the lanes of the wavefront that share the bucket (sorted order keeps them
+// adjacent), then one atomic per segment.  Inactive lanes pass key 0xFFFFFFFF.
+__device__ __forceinline__ void record_displacement(uint32_t* table, uint32_t key, float d) {
+    const int lane = threadIdx.x & 63;
+    uint32_t b = __float_as_uint(d);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ob = __shfl_up(b, off), ok = __shfl_up(key, off);
+        if (lane >= off && ok == key && ob > b) b = ob;
+    }
+    const uint32_t next = __shfl_down(key, 1);
+    if (key != 0xFFFFFFFFu && (lane == 63 || next != key) && b) atomicMax(&table[key], b);
+}
+
+// D_i: the largest recorded displacement in the 27 buckets around cell c.
+__device__ __forceinline__ float bucket_displacement(const uint32_t* __restrict__ table, int3 c,
+                                                     const HashGrid& g, uint32_t base) {
+    uint32_t m = 0;
+    for_each_bucket(c, g, [&](uint32_t key) {
+        const uint32_t v = table[base + key];
+        m = v > m ? v : m;
+    });
+    return __uint_as_float(m);
+}
 
 struct ListWriter {
     uint32_t* base;
@@ -473,6 +519,7 @@ __global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4
     const bool in = i < rg.end;
     const bool lists = NL.on != 0;
     const R lr2 = (R)P.list_r2;
+    if (lists) clear_table(NL.disp[0], NL.buckets);  // filled by the force pass
     unsigned long long cf_sum = 0, cb_sum = 0;
     if (!(rg.flags & kRangeBoundary)) {
         if (in) {
@@ -601,11 +648,13 @@ __global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const
     LoopState* st = B.states + rg.scene;
     const uint32_t i = rg.begin + threadIdx.x;
     float moved = 0.f;
+    uint32_t bucket = 0xFFFFFFFFu;
     if (i < rg.end) {
         const V4<R> self = xr[i];
         const V4<R> sv = vel[i];
         const V4<R> sn = nrm[i];
         const int3 c = cell_of4<R>(self, P.cell);
+        bucket = P.bucket_base + hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, P.grid);
         R vx = 0, vy = 0, vz = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
         float a_rho = 0.f, a_p = 0.f, a_div = 0.f, a_n = 0.f;
         const bool want = F.feat != nullptr;
@@ -680,16 +729,15 @@ __global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const
                                  a_div * P.div_scale);
         }
     }
-    wave_max_to(&st->max_disp[0], moved);
+    if (NL.on) record_displacement(NL.disp[0], bucket, moved);
 }
 
 // update_pressure (pcisph.cl:212-263): fluid mode on fluid ranges, boundary mode on
 // boundary ranges (the two launches of Fluid.cpp:263-280 are independent).
 // Also publishes x + p/rho^2 for the force pass and the per-iteration max dv.
-// Distances are taken at the predicted positions, so the step's lists (built at
-// x with radius (1+skin)h) hold every pair within h only while no predicted
-// position of the scene has moved more than skin*h/2 from x; past P.disp_limit
-// the pass walks the buckets instead.
+// Distances are taken at the predicted positions, so a wavefront walks the step's
+// lists (built at x with radius (1+skin)h) only where every lane's displacement test
+// passes (NeighborLists), else its 27 buckets.
 template <typename R, bool FAST>
 __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
                            const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
@@ -698,20 +746,35 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
                            V4<R>* __restrict__ xf, R* __restrict__ bp, V4<R>* __restrict__ bxf, NeighborLists NL) {
     using O = Ops<R, FAST>;
     __shared__ unsigned long long lds[2][8];
+    __shared__ unsigned int s_list_waves;
+    if (NL.on) clear_table(NL.disp[(iter + 1) & 1], NL.buckets);  // for the next prediction
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
     if (iteration_skipped<R>(st, iter, P)) return;
-    const bool lists = NL.on && __uint_as_float(st->max_disp[iter]) <= P.disp_limit;
-    if (threadIdx.x == 0 && (rg.flags & kRangeFirstOfScene) && lists) atomicAdd(&st->list_passes, 1u);
+    if (threadIdx.x == 0) s_list_waves = 0;
     const uint2* fc = fcells + P.bucket_base;
     const uint32_t i = rg.begin + threadIdx.x;
+    const bool boundary = (rg.flags & kRangeBoundary) != 0;
+    // lists for this wavefront if every lane's pairs are all in its list
+    bool ok = true;
+    V4<R> self{}, pr{};
+    int3 c{};
+    if (i < rg.end) {
+        self = boundary ? bpos[i] : xr[i];
+        pr = boundary ? self : xp[i];
+        c = cell_of4<R>(self, P.cell);
+        if (NL.on) {
+            const float d = boundary ? 0.f : displacement<R>(pr, self);
+            ok = d + bucket_displacement(NL.disp[iter & 1], c, P.grid, P.bucket_base) <= P.disp_limit;
+        }
+    }
+    const bool lists = NL.on && __all(ok);
+    __syncthreads();
+    if (lists && (threadIdx.x & 63) == 0) atomicAdd(&s_list_waves, 1u);
     unsigned long long m[2] = {0, 0};  // max dv, max expansion
-    if (!(rg.flags & kRangeBoundary)) {
+    if (!boundary) {
         if (i < rg.end) {
-            const V4<R> self = xr[i];
-            const V4<R> pr = xp[i];
-            const int3 c = cell_of4<R>(self, P.cell);
             R pd = 0;
             if (P.nb) {
                 const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
@@ -746,8 +809,6 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
         }
     } else if (i < rg.end) {
         const uint32_t b = i;
-        const V4<R> self = bpos[b];
-        const int3 c = cell_of4<R>(self, P.cell);
         R pd = 0;
         pd += binit[b];
         const uint32_t len = lists ? NL.bfcount[b] : kListOverflow;
@@ -768,6 +829,7 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
     }
     unsigned long long* const dst[2] = {&st->max_dv[iter], &st->max_exp[iter]};
     block_max_to<2>(m, dst, lds);
+    if (threadIdx.x == 0 && s_list_waves) atomicAdd(&st->list_waves, s_list_waves);
 }
 
 // Publication of iteration `iter` by the first workgroup of the pressure-force launch
@@ -792,7 +854,7 @@ __device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* 
         r.candidates[1] = st->boundary_candidates;
         r.iterations = iter + 1;
         r.warm_rejected = st->warm_rejected;
-        r.list_passes = st->list_passes;
+        r.list_waves = st->list_waves;
         r.pad = 0;
         res[s] = r;
         if (!iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations, P.rho0, P.threshold))
@@ -838,10 +900,12 @@ __global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restr
                                                          P.rho0, P.threshold);
     const uint32_t i = rg.begin + threadIdx.x;
     float moved = 0.f;
+    uint32_t bucket = 0xFFFFFFFFu;
     if (i < rg.end) {
         const V4<R> self = xf[i];
         const R sf = self.w;
         const int3 c = cell_of4<R>(self, P.cell);
+        bucket = P.bucket_base + hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, P.grid);
         R fx = 0, fy = 0, fz = 0;
         auto add = [&](const V4<R>& o) {
             R dx, dy, dz;
@@ -888,7 +952,7 @@ __global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restr
             if (bad) atomicOr(&st->nonfinite, 1u);
         }
     }
-    if (!last) wave_max_to(&st->max_disp[warm_pass ? 0 : iter + 1], moved);
+    if (!last && !warm_pass && NL.on) record_displacement(NL.disp[(iter + 1) & 1], bucket, moved);
 }
 
 // The network's output (pressure in units of p_ref, written by the inference kernel)
@@ -919,6 +983,7 @@ __global__ void k_pressure_dual(Batch<R> B, const uint2* __restrict__ bcells, co
                                 V4<R>* __restrict__ bxf, DualArrays<R> D) {
     using O = Ops<R, FAST>;
     __shared__ unsigned long long lds[4][8];
+    if (D.disp_next) clear_table(D.disp_next, D.buckets);  // for iteration 1's prediction
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;

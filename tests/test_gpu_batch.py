@@ -18,7 +18,7 @@ from tests.helpers import FLOAT_FIELDS, INT_FIELDS, run, small_scenes, snapshot
 pytestmark = pytest.mark.gpu
 
 STAT_KEYS = ("iterations", "converged", "max_density_variation", "max_density_error", "fluid_candidates",
-             "boundary_candidates", "warm_start", "list_passes")
+             "boundary_candidates", "warm_start", "list_waves")
 
 
 @pytest.fixture(scope="module")

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 EACH_STEP = [Field.FLUID_POSITIONS, Field.FLUID_VELOCITIES, Field.FLUID_PRESSURES, Field.FLUID_KEYS]
 STAT_KEYS = ("iterations", "converged", "max_density_variation", "fluid_candidates", "boundary_candidates",
-             "warm_start", "list_passes")
+             "warm_start", "list_waves")
 
 
 @pytest.fixture(scope="module")

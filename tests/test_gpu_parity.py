@@ -159,9 +159,10 @@ def test_sample_collection(gpu):
 
 @pytest.mark.parametrize("skin", [0.0, 1e-4, 0.2, 0.6])
 def test_neighbour_lists_are_exact(gpu, oracle_lib, skin):
-    """Neighbour lists off (0), on but invalid for every pressure pass (1e-4: the
-    predicted positions always move further than the skin allows), the default, and
-    a wide skin: the IEEE step is the oracle's, bit for bit."""
+    """Neighbour lists off (0), on but too tight for most of the pressure passes (1e-4:
+    the predicted positions move further than the skin allows, so nearly every
+    wavefront walks its buckets), the default, and a wide skin (pressure passes walk
+    the lists): the IEEE step is the oracle's, bit for bit."""
     scene = small_scenes()[1]
     print(f"case lists skin={skin}", flush=True)
     o = oracle_lib.OracleFluid()
@@ -171,6 +172,11 @@ def test_neighbour_lists_are_exact(gpu, oracle_lib, skin):
     assert [s["iterations"] for s in so] == [s["iterations"] for s in sg]
     for f in FLOAT_FIELDS:
         np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
+    waves = sum(s["list_waves"] for s in sg)
+    if skin == 0.0:
+        assert waves == 0
+    elif skin >= 0.2:
+        assert waves > 0
 
 
 def test_list_overflow_falls_back_to_buckets(gpu, oracle_lib):
