@@ -135,8 +135,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
-    ap.add_argument("--forms", default="plain,warm_start,batch_64",
-                    help="comma list of plain, warm_start (trained network), batch_64 (64-scene batch)")
+    ap.add_argument("--forms", default="plain,previous_pressure,warm_start,batch_64",
+                    help="comma list of plain (reference loop), previous_pressure (each step starts from "
+                         "the previous pressures), warm_start (trained network), batch_64 (64-scene batch)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--weights", default=os.path.join(ROOT, "pcisph_amd", "data", "warm_start.npy"))
     args = ap.parse_args()
@@ -176,6 +177,9 @@ def main() -> int:
                     print(f"bench: no trained network at {args.weights}; warm_start skipped", file=sys.stderr)
                     continue
                 fluid.mlp_set_params(np.load(args.weights))
+                fluid.set_warm_start(True)
+            elif form == "previous_pressure":
+                fluid.mlp_set_params(np.zeros(pcisph_amd.MLP_PARAMS, np.float32))
                 fluid.set_warm_start(True)
             print(f"bench: {form} {scene.name} N={n} Nb={scene.boundary_count} warmup={args.warmup} "
                   f"steps={args.steps}", file=sys.stderr, flush=True)

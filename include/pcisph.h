@@ -107,6 +107,14 @@ typedef struct pcisph_solver_options {
                                        whenever a launch argument changes; results are
                                        identical to 0 (launch by launch).  Default 4.  Stage
                                        profiling launches eagerly whatever this says. */
+    uint32_t work_order;            /* bit 0: fluid work items follow a per-step Morton order
+                                       of the cells (data and results stay in the reference's
+                                       sorted order; sums are unchanged), so a wavefront
+                                       covers a compact block of cells; bit 1: workgroups are
+                                       placed so that each of the 8 XCDs (own L2) works on one
+                                       contiguous part of that order.  Default 1 (the
+                                       placement measured slower on the 1M dam break). */
+    uint32_t reserved;
 } pcisph_solver_options;
 
 /* Per-step report (the reference only has a compiled-out timer, Fluid.cpp:319-326). */

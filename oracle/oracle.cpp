@@ -221,7 +221,7 @@ struct Sim {
     bool params_changed = true;
     bool boundary_updated = true;
     float threshold = 0.01f;
-    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 0};  // the oracle launches nothing: graph_iterations unused
+    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 0, 0, 0};  // launch options unused: nothing is launched
     // public particle state
     std::vector<T> b_pos, b_p;
     std::vector<T> f_pos, f_n, f_pred, f_rho, f_fo, f_v, f_p, f_fp, f_dv;

@@ -53,6 +53,8 @@ class SolverOptions(C.Structure):
         ("sort_digit_bits", C.c_uint32),
         ("neighbor_skin", C.c_float),
         ("graph_iterations", C.c_uint32),
+        ("work_order", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -179,9 +181,9 @@ class SolverBase:
 
     def set_solver_options(self, min_iterations=3, max_iterations=7, launch_boundary_init=0,
                            resort_boundary=0, block_size=0, sort_digit_bits=0, neighbor_skin=0.2,
-                           graph_iterations=4):
+                           graph_iterations=4, work_order=1):
         o = SolverOptions(min_iterations, max_iterations, launch_boundary_init, resort_boundary,
-                          block_size, sort_digit_bits, neighbor_skin, graph_iterations)
+                          block_size, sort_digit_bits, neighbor_skin, graph_iterations, work_order)
         self._call("set_solver_options", C.byref(o))
 
     def field_size(self, field: int) -> int:

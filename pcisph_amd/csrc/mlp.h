@@ -33,12 +33,13 @@ void mlp_gradients(const MlpLaunch& L, const float* params, const float* x, cons
 void mlp_adam(const MlpLaunch& L, float* params, const float* grad, float* m, float* v, uint32_t step, float lr,
               float beta1, float beta2, float eps);
 
-// Warm start: y[i] = max(0, net(feat[i])), the initial pressure in units of the
-// scene's p_ref.
+// Warm start: y[i] = net(feat[i]), the change of the particle's pressure from the
+// previous step's converged value, in units of the scene's p_ref.
 template <typename R>
 void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, uint32_t n, R* y);
-// Appends (feat[i], p[i] / p_ref) for i = phase, phase + stride, ... to the sample
-// store at slots base + i / stride (those below cap).
+// Appends (feat[i], p[i] / p_ref - feat[i][2]) -- the converged pressure's change from
+// the previous step's, whose p / p_ref is input 2 -- for i = phase, phase + stride, ...
+// to the sample store at slots base + i / stride (those below cap).
 template <typename R>
 void mlp_collect(const MlpLaunch& L, const float* feat, const R* p, uint32_t n, uint32_t stride, uint32_t phase,
                  float p_ref, float* store_x, float* store_t, uint64_t base, uint64_t cap);

@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_warm(const float* __restrict__
         const uint32_t row = row0 + (lane & 15);
         v4f z1[4], z2[4];
         const float out = tile_forward(s, load_x(x, row, n, g), z1, z2);
-        if (g == 0 && row < n) y[row] = (R)fmaxf(out, 0.f);
+        if (g == 0 && row < n) y[row] = (R)out;
     }
 }
 
@@ -345,7 +345,7 @@ __global__ void k_collect(const float* __restrict__ feat, const R* __restrict__ 
     float4* dst = reinterpret_cast<float4*>(sx + slot * kMlpIn);
     dst[0] = src[0];
     dst[1] = src[1];
-    st[slot] = (float)p[i] / p_ref;
+    st[slot] = (float)p[i] / p_ref - feat[(size_t)i * kMlpIn + 2];  // change from the previous pressure
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser

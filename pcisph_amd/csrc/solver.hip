@@ -352,6 +352,10 @@ class Engine final : public EngineBase {
         dfree(A_.NL.bfcount);
         dfree(A_.NL.disp[0]);
         dfree(A_.NL.disp[1]);
+        for (int k = 0; k < 2; ++k) {
+            dfree(okeys_[k]);
+            dfree(ovals_[k]);
+        }
         dfree(D_.xp0);
         dfree(D_.xf0);
         dfree(D_.p0);
@@ -498,7 +502,7 @@ class Engine final : public EngineBase {
     }
 
     void translate_boundaries(const float* shifts, uint32_t block) override {
-        prepare_blocks(block);
+        prepare_blocks(block, blocks_for_ & 0x80000000u);
         const size_t S = n_.size();
         if (shift_cap_ < S) {
             HIP_CHECK(hipStreamSynchronize(stream_));
@@ -683,13 +687,30 @@ class Engine final : public EngineBase {
 
     // The block table for workgroups of `block` threads: every scene's fluid
     // particles in ranges of `block`, then every scene's boundary particles.
-    void prepare_blocks(uint32_t block) {
-        if (blocks_for_ == block) return;
+    //
+    // With `xcd` the fluid ranges are placed for the 8 XCDs, to which workgroups are
+    // dispatched round-robin (workgroup w runs on XCD w mod 8): XCD x gets one
+    // contiguous run of ranges, so in the spatially coherent work order each XCD's L2
+    // holds one compact part of the domain.
+    void prepare_blocks(uint32_t block, bool xcd) {
+        const uint32_t want = block | (xcd ? 0x80000000u : 0u);
+        if (blocks_for_ == want) return;
         std::vector<BlockRange> t;
         for (uint32_t s = 0; s < n_.size(); ++s)
             for (uint32_t b = f_off_[s]; b < f_off_[s + 1]; b += block)
                 t.push_back({s, b, std::min(b + block, f_off_[s + 1]), b == f_off_[s] ? kRangeFirstOfScene : 0u});
         const uint32_t fluid_blocks = (uint32_t)t.size();
+        if (xcd && fluid_blocks > 8) {
+            constexpr uint32_t kXcds = 8;
+            std::vector<BlockRange> placed(fluid_blocks);
+            uint32_t start = 0;
+            for (uint32_t x = 0; x < kXcds; ++x) {
+                const uint32_t count = x < fluid_blocks ? (fluid_blocks - 1 - x) / kXcds + 1 : 0;
+                for (uint32_t q = 0; q < count; ++q) placed[q * kXcds + x] = t[start + q];
+                start += count;
+            }
+            std::copy(placed.begin(), placed.end(), t.begin());
+        }
         for (uint32_t s = 0; s < nb_.size(); ++s)
             for (uint32_t b = b_off_[s]; b < b_off_[s + 1]; b += block)
                 t.push_back({s, b, std::min(b + block, b_off_[s + 1]), kRangeBoundary});
@@ -701,7 +722,7 @@ class Engine final : public EngineBase {
         HIP_CHECK(hipStreamSynchronize(stream_));
         fluid_blocks_ = fluid_blocks;
         boundary_blocks_ = (uint32_t)t.size() - fluid_blocks;
-        blocks_for_ = block;
+        blocks_for_ = want;
     }
 
     // The scene table of this step; uploaded when it differs from the last one.
@@ -782,7 +803,8 @@ class Engine final : public EngineBase {
                               (const void*)D_.p0, (const void*)D_.dv0, (const void*)D_.bp0, (const void*)D_.bxf0})
             ptr(p);
         const Batch<R>& B = A_.batch;
-        ptr(B.scenes), ptr(B.blocks), ptr(B.states);
+        ptr(B.scenes), ptr(B.blocks), ptr(B.states), ptr(B.order), ptr(okeys_[0]), ptr(okeys_[1]),
+            ptr(ovals_[0]), ptr(ovals_[1]);
         put(B.scene_count), put(B.fluid_blocks), put(B.boundary_blocks), put(B.n_total), put(B.nb_total);
         const NeighborLists& NL = A_.NL;
         ptr(NL.f), ptr(NL.b), ptr(NL.bf), ptr(NL.fcount), ptr(NL.bcount), ptr(NL.bfcount);
@@ -803,10 +825,19 @@ class Engine final : public EngineBase {
         const bool warm = w.params != nullptr;
         const bool features = warm || w.collect_stride;
         const uint32_t maxit = o.max_iterations, minit = std::max(o.min_iterations, 1u);
-        prepare_blocks(L.block);
+        prepare_blocks(L.block, (o.work_order & 2) != 0);
         setup_lists(o.neighbor_skin);
         prepare_scenes(scenes, o);
-        A_.batch = Batch<R>{scene_table_, block_table_, states_, S, fluid_blocks_, boundary_blocks_, A_.n, A_.nb, 0};
+        // work order: Morton codes with `obits` bits per axis below the scene index
+        const uint32_t sbits = key_bits_for(S);
+        const uint32_t obits = std::min(7u, sbits < 32 ? (32u - sbits) / 3u : 0u);
+        const bool ordered = (o.work_order & 1) && obits >= 2 && A_.n > 1;
+        const uint32_t order_bits = ordered ? 3 * obits + sbits : 0;
+        if (ordered) ensure_order_buffers();
+        const uint32_t order_passes = (order_bits + digit - 1) / digit;
+        const uint32_t* order = ordered ? ovals_[order_passes & 1] : nullptr;
+        A_.batch = Batch<R>{scene_table_, block_table_, states_, order, S, fluid_blocks_, boundary_blocks_,
+                            A_.n, A_.nb, 0};
         if (features) ensure_warm_buffers(warm);
         D_.disp_next = A_.NL.on ? A_.NL.disp[1] : nullptr;
         D_.buckets = A_.NL.buckets;
@@ -852,6 +883,12 @@ class Engine final : public EngineBase {
             prof_.timed(PCISPH_STAGE_REORDER, stream_, -1, [&] {
                 DataLaunch<R>::reorder(L, A_.n, A_.keys[fluid_sorted_], A_.vals[fluid_sorted_], A_.pos_in,
                                        A_.vel_in, A_.pos, A_.vel, A_.cells);
+                if (ordered) {
+                    DataLaunch<R>::morton(L, A_.batch, A_.pos, obits, okeys_[0], ovals_[0]);
+                    const int res = radix_sort(L, okeys_, ovals_, A_.n, order_bits, digit, sort_scratch_);
+                    if (res != (int)(order_passes & 1))
+                        throw Error(PCISPH_ERR_INTERNAL, "work-order sort ended in an unexpected buffer");
+                }
             });
             prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
             prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, A_); });
@@ -990,6 +1027,15 @@ class Engine final : public EngineBase {
         NL.on = 1;
     }
 
+    void ensure_order_buffers() {
+        if (order_n_ == A_.n && okeys_[0]) return;
+        for (int k = 0; k < 2; ++k) {
+            dalloc(okeys_[k], std::max(A_.n, 1u), stream_);
+            dalloc(ovals_[k], std::max(A_.n, 1u), stream_);
+        }
+        order_n_ = A_.n;
+    }
+
     void ensure_warm_buffers(bool dual) {
         const uint32_t n = A_.n, nb = A_.nb;
         if (feat_n_ != n) {
@@ -1043,6 +1089,9 @@ class Engine final : public EngineBase {
     float* feat_ = nullptr;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
     float list_skin_ = 0.f;
+    uint32_t* okeys_[2] = {nullptr, nullptr};  // work-order sort: keys
+    uint32_t* ovals_[2] = {nullptr, nullptr};  // and slots (the result is the order)
+    uint32_t order_n_ = 0;
     uint32_t list_buckets_ = 0;
     uint32_t* ctl_ = nullptr;  // device word: the tag of the step in flight
     hipGraphExec_t prologue_graph_ = nullptr;
@@ -1105,7 +1154,7 @@ struct pcisph_solver {
     bool own_stream = false;
     std::vector<Scene> scenes = std::vector<Scene>(1);
     uint32_t current = 0;  // the scene that setters, uploads and downloads address
-    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 4};
+    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 4, 1, 0};
     int precision = kPrecisionFast;
     std::unique_ptr<EngineBase> eng;
     std::string err;
@@ -1275,6 +1324,7 @@ int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "sort_digit_bits must be in 4..8");
         if (!(opt->neighbor_skin >= 0.f && opt->neighbor_skin <= 2.f))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "neighbor_skin must be in [0, 2]");
+        if (opt->work_order > 3) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..3");
         s->opt = *opt;
     });
 }

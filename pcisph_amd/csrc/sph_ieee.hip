@@ -56,6 +56,29 @@ __global__ void k_translate(V4<R>* __restrict__ p, uint32_t n, R dx, R dy, R dz)
     p[i] = v;
 }
 
+// The low `bits` bits of v spread to every third bit (bit k -> bit 3k).
+__device__ __forceinline__ uint32_t spread3(uint32_t v, uint32_t bits) {
+    uint32_t out = 0;
+    for (uint32_t k = 0; k < bits; ++k) out |= ((v >> k) & 1u) << (3 * k);
+    return out;
+}
+
+// Work-order keys of the sorted fluid particles: scene index above the Morton code
+// of the particle's cell (coordinates taken modulo 2^bits), value = the slot.
+template <typename R>
+__global__ void k_morton(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
+                         const V4<R>* __restrict__ pos, uint32_t bits, uint32_t* __restrict__ keys,
+                         uint32_t* __restrict__ vals) {
+    const BlockRange rg = blocks[blockIdx.x];
+    const uint32_t t = rg.begin + threadIdx.x;
+    if (t >= rg.end) return;
+    const int3 c = cell_of4<R>(pos[t], scenes[rg.scene].cell);
+    const uint32_t code = spread3((uint32_t)c.x, bits) | (spread3((uint32_t)c.y, bits) << 1) |
+                          (spread3((uint32_t)c.z, bits) << 2);
+    keys[t] = (rg.scene << (3 * bits)) | code;
+    vals[t] = t;
+}
+
 // Every boundary range moved by its scene's shift[3 * scene ..].
 template <typename R>
 __global__ void k_translate_scenes(const BlockRange* __restrict__ blocks, const float* __restrict__ shift,
@@ -72,6 +95,14 @@ __global__ void k_translate_scenes(const BlockRange* __restrict__ blocks, const 
 }
 
 }  // namespace
+
+template <typename R>
+void DataLaunch<R>::morton(const Launch& L, const Batch<R>& B, const V4<R>* pos, uint32_t bits, uint32_t* keys,
+                           uint32_t* vals) {
+    if (!B.fluid_blocks) return;
+    hipLaunchKernelGGL((k_morton<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B.scenes, B.blocks, pos, bits,
+                       keys, vals);
+}
 
 template <typename R>
 void DataLaunch<R>::translate_scenes(const Launch& L, const Batch<R>& B, const float* shift, V4<R>* bpos) {
@@ -99,7 +130,7 @@ template <typename R>
 void DataLaunch<R>::warm_apply(const Launch& L, const DeviceArrays<R>& A) {
     if (!A.batch.fluid_blocks) return;
     hipLaunchKernelGGL((k_warm_apply<R>), dim3(A.batch.fluid_blocks), dim3(L.block), 0, L.stream, A.batch, A.xr,
-                       A.p, A.xf);
+                       A.vel, A.p, A.xf);
 }
 
 template <typename R>

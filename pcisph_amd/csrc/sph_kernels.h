@@ -136,16 +136,32 @@ struct FeatureArgs {
 };
 
 // The batch as every step kernel sees it.
+//
+// Work order.  Fluid data stays in the reference's sorted (cell-hash) order, which
+// the bucket walks need and the outputs keep; but consecutive hash keys are cells
+// scattered over the whole domain, so a wavefront of consecutive particles would
+// gather from ~8 unrelated neighbourhoods.  Fluid work items therefore go through
+// `order`: slot t of a fluid range works on particle order[t], a Morton order of the
+// cells within each scene, so a wavefront covers a compact block of cells and its
+// gathers share cache lines.  Per-particle sums still visit the candidates in the
+// reference's bucket order.  Neighbour lists and their counts are kept per slot t,
+// so a wavefront reads its lists coalesced.
 template <typename R>
 struct Batch {
     const SphParams<R>* scenes;   // [S]
     const BlockRange* blocks;     // fluid ranges [0, fluid_blocks), then boundary ranges
     LoopState* states;            // [S]
+    const uint32_t* order;        // fluid slot -> particle, or null (slot t is particle t)
     uint32_t scene_count;
     uint32_t fluid_blocks, boundary_blocks;
     uint32_t n_total, nb_total;   // list strides
     uint32_t pad;
 };
+
+template <typename R>
+__device__ __forceinline__ uint32_t particle_of(const Batch<R>& B, uint32_t t) {
+    return B.order ? B.order[t] : t;
+}
 
 template <typename R>
 struct Bits;
@@ -515,20 +531,21 @@ __global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
     const uint2* fc = fcells + P.bucket_base;
-    const uint32_t i = rg.begin + threadIdx.x;
-    const bool in = i < rg.end;
+    const uint32_t t = rg.begin + threadIdx.x;
+    const bool in = t < rg.end;
     const bool lists = NL.on != 0;
     const R lr2 = (R)P.list_r2;
     if (lists) clear_table(NL.disp[0], NL.buckets);  // filled by the force pass
     unsigned long long cf_sum = 0, cb_sum = 0;
     if (!(rg.flags & kRangeBoundary)) {
         if (in) {
+            const uint32_t i = particle_of(B, t);
             const uint2* bc = bcells + P.bucket_base;
             const V4<R> self = pos[i];
             const int3 c = cell_of4<R>(self, P.cell);
             R rho = 0;
             uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
-            ListWriter wb{NL.b, B.n_total, i, NL.cap_b, 0}, wf{NL.f, B.n_total, i, NL.cap_f, 0};
+            ListWriter wb{NL.b, B.n_total, t, NL.cap_b, 0}, wf{NL.f, B.n_total, t, NL.cap_f, 0};
             if (P.nb)
                 walk_buckets<R>(bc, c, P.grid, [&](uint32_t j) { return bpos[j]; },
                                 [&](const V4<R>& o, uint32_t j) {
@@ -554,14 +571,14 @@ __global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4
             xr[i] = out;
             if (counts) counts[i] = make_uint4(cf, cb, nf, nb);
             if (lists) {
-                NL.bcount[i] = P.nb ? wb.result() : 0u;
-                NL.fcount[i] = wf.result();
+                NL.bcount[t] = P.nb ? wb.result() : 0u;
+                NL.fcount[t] = wf.result();
             }
             cf_sum = cf;
             cb_sum = cb;
         }
     } else if (in) {
-        const uint32_t b = i;
+        const uint32_t b = t;
         bp[b] = 0;
         V4<R> o = bpos[b];
         o.w = 0;
@@ -592,8 +609,9 @@ __global__ void k_normal(Batch<R> B, const uint2* __restrict__ fcells, const V4<
                          V4<R>* __restrict__ nrm, NeighborLists NL) {
     using O = Ops<R, FAST>;
     const BlockRange rg = B.blocks[blockIdx.x];
-    const uint32_t i = rg.begin + threadIdx.x;
-    if (i >= rg.end) return;
+    const uint32_t t = rg.begin + threadIdx.x;
+    if (t >= rg.end) return;
+    const uint32_t i = particle_of(B, t);
     const SphParams<R> P = B.scenes[rg.scene];
     const V4<R> self = xr[i];
     const int3 c = cell_of4<R>(self, P.cell);
@@ -607,8 +625,8 @@ __global__ void k_normal(Batch<R> B, const uint2* __restrict__ fcells, const V4<
         ny += O::div(w * dy, o.w);
         nz += O::div(w * dz, o.w);
     };
-    const uint32_t len = NL.on ? NL.fcount[i] : kListOverflow;
-    walk<R>(NL.f, len, B.n_total, i, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xr[j]; }, body);
+    const uint32_t len = NL.on ? NL.fcount[t] : kListOverflow;
+    walk<R>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xr[j]; }, body);
     V4<R> out;
     out.x = nx * P.normal_scale;
     out.y = ny * P.normal_scale;
@@ -645,11 +663,11 @@ __global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const
     using O = Ops<R, FAST>;
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
-    LoopState* st = B.states + rg.scene;
-    const uint32_t i = rg.begin + threadIdx.x;
+    const uint32_t t = rg.begin + threadIdx.x;
     float moved = 0.f;
     uint32_t bucket = 0xFFFFFFFFu;
-    if (i < rg.end) {
+    if (t < rg.end) {
+        const uint32_t i = particle_of(B, t);
         const V4<R> self = xr[i];
         const V4<R> sv = vel[i];
         const V4<R> sn = nrm[i];
@@ -698,8 +716,8 @@ __global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const
                 uz += corr * (sn.z - on.z);
             }
         };
-        const uint32_t len = NL.on ? NL.fcount[i] : kListOverflow;
-        walk<R>(NL.f, len, B.n_total, i, fcells + P.bucket_base, c, P.grid, load, body);
+        const uint32_t len = NL.on ? NL.fcount[t] : kListOverflow;
+        walk<R>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, load, body);
         const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
         const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
         const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
@@ -754,13 +772,14 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
     if (iteration_skipped<R>(st, iter, P)) return;
     if (threadIdx.x == 0) s_list_waves = 0;
     const uint2* fc = fcells + P.bucket_base;
-    const uint32_t i = rg.begin + threadIdx.x;
+    const uint32_t t = rg.begin + threadIdx.x;
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
+    const uint32_t i = boundary || t >= rg.end ? t : particle_of(B, t);
     // lists for this wavefront if every lane's pairs are all in its list
     bool ok = true;
     V4<R> self{}, pr{};
     int3 c{};
-    if (i < rg.end) {
+    if (t < rg.end) {
         self = boundary ? bpos[i] : xr[i];
         pr = boundary ? self : xp[i];
         c = cell_of4<R>(self, P.cell);
@@ -774,19 +793,19 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
     if (lists && (threadIdx.x & 63) == 0) atomicAdd(&s_list_waves, 1u);
     unsigned long long m[2] = {0, 0};  // max dv, max expansion
     if (!boundary) {
-        if (i < rg.end) {
+        if (t < rg.end) {
             R pd = 0;
             if (P.nb) {
-                const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
-                walk<R>(NL.b, len, B.n_total, i, bcells + P.bucket_base, c, P.grid,
+                const uint32_t len = lists ? NL.bcount[t] : kListOverflow;
+                walk<R>(NL.b, len, B.n_total, t, bcells + P.bucket_base, c, P.grid,
                         [&](uint32_t j) { return bpos[j]; },
                         [&](const V4<R>& o, uint32_t) {
                             R dx, dy, dz;
                             pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
                         });
             }
-            const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
-            walk<R>(NL.f, len, B.n_total, i, fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
+            const uint32_t len = lists ? NL.fcount[t] : kListOverflow;
+            walk<R>(NL.f, len, B.n_total, t, fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
                     [&](const V4<R>& o, uint32_t) {
                         R dx, dy, dz;
                         pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
@@ -807,8 +826,8 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
             xf[i] = out;
             m[0] = Bits<R>::of(v);
         }
-    } else if (i < rg.end) {
-        const uint32_t b = i;
+    } else if (t < rg.end) {
+        const uint32_t b = t;
         R pd = 0;
         pd += binit[b];
         const uint32_t len = lists ? NL.bfcount[b] : kListOverflow;
@@ -898,10 +917,11 @@ __global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restr
     if (!warm_pass && iteration_skipped<R>(st, iter, P)) return;
     const bool last = !warm_pass && iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations,
                                                          P.rho0, P.threshold);
-    const uint32_t i = rg.begin + threadIdx.x;
+    const uint32_t t = rg.begin + threadIdx.x;
     float moved = 0.f;
     uint32_t bucket = 0xFFFFFFFFu;
-    if (i < rg.end) {
+    if (t < rg.end) {
+        const uint32_t i = particle_of(B, t);
         const V4<R> self = xf[i];
         const R sf = self.w;
         const int3 c = cell_of4<R>(self, P.cell);
@@ -922,12 +942,12 @@ __global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restr
         // positions here are x, where the lists are exact
         const bool lists = NL.on != 0;
         if (P.nb) {
-            const uint32_t len = lists ? NL.bcount[i] : kListOverflow;
-            walk<R>(NL.b, len, B.n_total, i, bcells + P.bucket_base, c, P.grid,
+            const uint32_t len = lists ? NL.bcount[t] : kListOverflow;
+            walk<R>(NL.b, len, B.n_total, t, bcells + P.bucket_base, c, P.grid,
                     [&](uint32_t j) { return bxf[j]; }, [&](const V4<R>& o, uint32_t) { add(o); });
         }
-        const uint32_t len = lists ? NL.fcount[i] : kListOverflow;
-        walk<R>(NL.f, len, B.n_total, i, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xf[j]; },
+        const uint32_t len = lists ? NL.fcount[t] : kListOverflow;
+        walk<R>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xf[j]; },
                 [&](const V4<R>& o, uint32_t) { add(o); });
         V4<R> f;
         f.x = fx * P.pf_scale;
@@ -955,15 +975,18 @@ __global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restr
     if (!last && !warm_pass && NL.on) record_displacement(NL.disp[(iter + 1) & 1], bucket, moved);
 }
 
-// The network's output (pressure in units of p_ref, written by the inference kernel)
-// made a pressure and the force pass's p/rho^2; fluid ranges.
+// The network's start: p = max(0, p_prev + y p_ref), y the inference kernel's output
+// (held in p) and p_prev the particle's converged pressure of the previous step
+// (the velocity's w); then the force pass's p/rho^2.  A network that outputs 0
+// starts every particle from its previous pressure.  Fluid ranges.
 template <typename R>
-__global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, R* __restrict__ p, V4<R>* __restrict__ xf) {
+__global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, const V4<R>* __restrict__ vel,
+                             R* __restrict__ p, V4<R>* __restrict__ xf) {
     const BlockRange rg = B.blocks[blockIdx.x];
     const uint32_t i = rg.begin + threadIdx.x;
     if (i >= rg.end) return;
     const float p_ref = B.scenes[rg.scene].p_ref;
-    const R pi = (R)((float)p[i] * p_ref);
+    const R pi = fmax(R(0), vel[i].w + (R)((float)p[i] * p_ref));
     p[i] = pi;
     V4<R> o = xr[i];
     o.w = pi / (o.w * o.w);
@@ -988,10 +1011,12 @@ __global__ void k_pressure_dual(Batch<R> B, const uint2* __restrict__ bcells, co
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
     const uint2* fc = fcells + P.bucket_base;
-    const uint32_t i = rg.begin + threadIdx.x;
+    const uint32_t t = rg.begin + threadIdx.x;
+    const bool boundary = (rg.flags & kRangeBoundary) != 0;
+    const uint32_t i = boundary || t >= rg.end ? t : particle_of(B, t);
     unsigned long long m[4] = {0, 0, 0, 0};  // dv warm, exp warm, dv plain, exp plain
-    if (!(rg.flags & kRangeBoundary)) {
-        if (i < rg.end) {
+    if (!boundary) {
+        if (t < rg.end) {
             const V4<R> self = xr[i];
             const V4<R> pw = xp[i], pz = D.xp0[i];
             const int3 c = cell_of4<R>(self, P.cell);

@@ -73,7 +73,7 @@ struct DataLaunch {
     // keys = bucket_base + hash of every fluid (boundary = false) or boundary particle
     static void hash(const Launch& L, const Batch<R>& B, bool boundary, const V4<R>* pos, uint32_t* keys,
                      uint32_t* vals);
-    // p = network output * p_ref of the scene, xf = (x, p / rho^2)
+    // p = max(0, previous pressure + network output * p_ref of the scene), xf = (x, p / rho^2)
     static void warm_apply(const Launch& L, const DeviceArrays<R>& A);
     static void reorder(const Launch& L, uint32_t n, const uint32_t* keys, const uint32_t* src,
                         const V4<R>* pin, const V4<R>* vin, V4<R>* pout, V4<R>* vout, uint2* cells);
@@ -90,6 +90,10 @@ struct DataLaunch {
                                 uint32_t offset);
     // p[i].xyz += (dx, dy, dz) in precision R
     static void translate(const Launch& L, V4<R>* p, uint32_t n, float dx, float dy, float dz);
+    // work-order keys (scene, Morton code of the cell with `bits` bits per axis) and
+    // slot values of the sorted fluid particles
+    static void morton(const Launch& L, const Batch<R>& B, const V4<R>* pos, uint32_t bits, uint32_t* keys,
+                       uint32_t* vals);
     // every scene's boundary positions += shift[3 s .. 3 s + 2] (device array)
     static void translate_scenes(const Launch& L, const Batch<R>& B, const float* shift, V4<R>* bpos);
 };

@@ -159,6 +159,7 @@ int main(int argc, char** argv) {
     check(s, pcisph_set_fluid_count(s, (uint32_t)(sc.fluid.size() / 3)), "set_fluid_count");
     check(s, pcisph_set_boundary_count(s, (uint32_t)(sc.boundary.size() / 3)), "set_boundary_count");
     pcisph_solver_options opt{};
+    check(s, pcisph_get_solver_options(s, &opt), "get_solver_options");  // library defaults
     opt.min_iterations = sc.min_iterations;
     opt.max_iterations = max_it ? max_it : sc.max_iterations;
     check(s, pcisph_set_solver_options(s, &opt), "set_solver_options");

@@ -1,13 +1,15 @@
 """Learned warm start of the pressure-correction loop: data generation, training and
 evaluation, all on the device through the C ABI (pcisph_samples_*, pcisph_mlp_*).
 
-The network (8 -> 64 SiLU -> 64 SiLU -> 1) predicts each fluid particle's converged
-pressure (in units of p_ref = delta * rho0 / 100) from local features the force pass
-gathers.  Samples are (features, converged pressure) pairs that the plain solver
-writes while it runs; training is Adam on random batches of them with the project's
-own MFMA forward/backward kernels.  At run time the loop starts from the predicted
-pressure, keeps it only if its first residual beats the plain start's, and still
-iterates to the same density-error bound.
+The network (8 -> 64 SiLU -> 64 SiLU -> 1) predicts, from local features the force
+pass gathers, how far each fluid particle's converged pressure moves from its
+previous step's value (in units of p_ref = delta * rho0 / 100).  The loop starts from
+max(0, p_prev + prediction * p_ref) -- a network that outputs zero is the classic
+previous-pressure start -- keeps that start only if its first residual beats the
+plain (zero-pressure) start's, and still iterates to the same density-error bound.
+Samples are (features, converged change) pairs the solver writes while it runs;
+training is Adam on random batches of them with the project's own MFMA
+forward/backward kernels.
 """
 from __future__ import annotations
 
@@ -25,9 +27,15 @@ def init_params(seed: int = 0) -> np.ndarray:
     return np.concatenate([w1.ravel(), np.zeros(64), w2.ravel(), np.zeros(64), w3, [0.0]]).astype(np.float32)
 
 
-def collect(fluid, steps: int, stride: int, capacity: int) -> int:
-    """Run `steps` plain steps storing every `stride`-th particle's sample."""
-    fluid.set_warm_start(False)
+def zero_params() -> np.ndarray:
+    """The network that outputs 0: the previous-pressure start."""
+    return np.zeros(N_PARAMS, np.float32)
+
+
+def collect(fluid, steps: int, stride: int, capacity: int, warm: bool = False) -> int:
+    """Run `steps` steps (warm-started with the loaded network if `warm`) storing every
+    `stride`-th particle's sample."""
+    fluid.set_warm_start(warm)
     fluid.samples_reserve(capacity)
     fluid.samples_collect(stride)
     fluid.run(steps, collect=False)
@@ -60,4 +68,4 @@ def mean_iterations(fluid, steps: int, warm: bool) -> dict:
             "warm_start_kept": float(used.mean()) if warm else 0.0}
 
 
-__all__ = ["init_params", "collect", "train", "mean_iterations", "N_PARAMS"]
+__all__ = ["init_params", "zero_params", "collect", "train", "mean_iterations", "N_PARAMS"]

@@ -44,7 +44,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(SimulationParams) == 80
     assert SimulationParams.bucket_count.offset == 28
     assert SimulationParams.density_variation_scaling_factor.offset == 76
-    assert ctypes.sizeof(SolverOptions) == 32
+    assert ctypes.sizeof(SolverOptions) == 40
     assert ctypes.sizeof(StepStats) == 48
     text = open(os.path.join(ROOT, "include", "pcisph.h")).read()
     for f in Field:
