@@ -63,7 +63,28 @@ def cpu_baseline(seconds: float, max_steps: int) -> dict:
                       f"{threads} threads, mean {np.mean(its):.2f} iterations/step"}
 
 
-def roofline_of(times: dict, n: int, cf: float, cb: float) -> tuple[dict, dict]:
+# HBM-side bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
+# the same counters on a copy of known size) for the plain 1M dam break.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_dam_break_1m_plain.json")
+TRAFFIC_KERNELS = {"density": "k_density", "normal": "k_normal", "force_init": "k_force_init",
+                   "pressure": "k_pressure", "pressure_force": "k_pressure_force"}
+
+
+def measured_traffic(stage: str):
+    """Measured bytes per launch of a stage's kernel, or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            kernels = json.load(fh)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    want = TRAFFIC_KERNELS.get(stage)
+    for name, v in kernels.items():
+        if name.split("::")[-1].split("<")[0] == want:
+            return v["traffic_bytes"]
+    return None
+
+
+def roofline_of(times: dict, n: int, cf: float, cb: float, traffic: bool = False) -> tuple[dict, dict]:
     """Per-stage device times and the roofline figure of the dominant neighbour kernel."""
     stage_ms = {STAGES[i]: {"ms_total": round(times["ms"][i], 3), "launches": times["launches"][i],
                             "us_per_launch": round(1e3 * times["ms"][i] / times["launches"][i], 2)
@@ -75,12 +96,13 @@ def roofline_of(times: dict, n: int, cf: float, cb: float) -> tuple[dict, dict]:
     bytes_per_launch = STAGE_BYTES[dom](cf, cb) * n
     achieved = bytes_per_launch / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom) if traffic else None,
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
     return stage_ms, roofline
 
 
-def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None) -> dict:
+def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, traffic: bool = False) -> dict:
     """`warmup` untimed then `steps` timed steps (every scene of a batch), HIP-event
     stage times over the timed region when `profile`."""
     shift = None
@@ -121,7 +143,7 @@ def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None) -
     if batch is not None:
         out["mean_iterations_slowest_scene"] = round(float(its.max(axis=1).mean()), 3)
     if times is not None:
-        out["stage_times"], out["roofline"] = roofline_of(times, n, cf, cb)
+        out["stage_times"], out["roofline"] = roofline_of(times, n, cf, cb, traffic)
     return out
 
 
@@ -140,11 +162,25 @@ def main() -> int:
                          "the previous pressures), warm_start (trained network), batch_64 (64-scene batch)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--weights", default=os.path.join(ROOT, "pcisph_amd", "data", "warm_start.npy"))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device five times (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
               file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}), flush=True)
+        return 0
 
     import pcisph_amd
     from pcisph_amd import scenes
@@ -183,7 +219,8 @@ def main() -> int:
                 fluid.set_warm_start(True)
             print(f"bench: {form} {scene.name} N={n} Nb={scene.boundary_count} warmup={args.warmup} "
                   f"steps={args.steps}", file=sys.stderr, flush=True)
-            r = measure(fluid, args.steps, args.warmup, n, not args.no_profile)
+            r = measure(fluid, args.steps, args.warmup, n, not args.no_profile,
+                        traffic=form == "plain" and args.workload == "dam_break_1m")
         fluid.close()
         results[form] = r
         print(f"bench: {form} {json.dumps({k: v for k, v in r.items() if k != 'stage_times'})}",
