@@ -102,19 +102,23 @@ def roofline_of(times: dict, n: int, cf: float, cb: float, traffic: bool = False
     return stage_ms, roofline
 
 
-def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, traffic: bool = False) -> dict:
+def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, traffic: bool = False,
+            scene=None) -> dict:
     """`warmup` untimed then `steps` timed steps (every scene of a batch), HIP-event
-    stage times over the timed region when `profile`."""
+    stage times over the timed region when `profile`.  Moving walls (sloshing scenes)
+    are translated before every step."""
+    from pcisph_amd import scenes
     shift = None
     if batch is not None and any(sc.meta.get("slosh") for sc in batch):
-        from pcisph_amd import scenes
-        shift = lambda k: [scenes.boundary_shift(sc, k) for sc in batch]  # noqa: E731
+        shift = lambda k: fluid.translate_boundaries([scenes.boundary_shift(sc, k) for sc in batch])  # noqa: E731
+    elif scene is not None and scene.meta.get("slosh"):
+        shift = lambda k: fluid.translate_boundary(*scenes.boundary_shift(scene, k))  # noqa: E731
 
     def go(count, start):
         out = []
         for k in range(count):
             if shift:
-                fluid.translate_boundaries(shift(start + k))
+                shift(start + k)
             out.append(fluid.step_scenes() if batch is not None else [fluid.step()])
         return out
 
@@ -157,9 +161,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
-    ap.add_argument("--forms", default="plain,previous_pressure,warm_start,batch_64",
+    ap.add_argument("--forms", default="plain,previous_pressure,warm_start,batch_64,sloshing_plain,sloshing_warm",
                     help="comma list of plain (reference loop), previous_pressure (each step starts from "
-                         "the previous pressures), warm_start (trained network), batch_64 (64-scene batch)")
+                         "the previous pressures), warm_start (trained network), batch_64 (64-scene batch), "
+                         "sloshing_plain / sloshing_warm (the 1M block in a tank oscillating along x, "
+                         "0.5 m at 0.8 Hz; plain loop / trained network)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--weights", default=os.path.join(ROOT, "pcisph_amd", "data", "warm_start.npy"))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -206,9 +212,11 @@ def main() -> int:
             r.update({"scenes": len(batch), "fluid_particles": total,
                       "boundary_particles": sum(sc.boundary_count for sc in batch)})
         else:
+            sc = scenes.sloshing_tank(0.5, 0.8, getattr(scenes, args.workload)()) \
+                if form.startswith("sloshing") else scene
             fluid = pcisph_amd.Fluid(device=0, precision=prec)
-            fluid.load_scene(scene)
-            if form == "warm_start":
+            fluid.load_scene(sc)
+            if form in ("warm_start", "sloshing_warm"):
                 if not os.path.exists(args.weights):
                     print(f"bench: no trained network at {args.weights}; warm_start skipped", file=sys.stderr)
                     continue
@@ -217,10 +225,10 @@ def main() -> int:
             elif form == "previous_pressure":
                 fluid.mlp_set_params(np.zeros(pcisph_amd.MLP_PARAMS, np.float32))
                 fluid.set_warm_start(True)
-            print(f"bench: {form} {scene.name} N={n} Nb={scene.boundary_count} warmup={args.warmup} "
+            print(f"bench: {form} {sc.name} N={n} Nb={sc.boundary_count} warmup={args.warmup} "
                   f"steps={args.steps}", file=sys.stderr, flush=True)
             r = measure(fluid, args.steps, args.warmup, n, not args.no_profile,
-                        traffic=form == "plain" and args.workload == "dam_break_1m")
+                        traffic=form == "plain" and args.workload == "dam_break_1m", scene=sc)
         fluid.close()
         results[form] = r
         print(f"bench: {form} {json.dumps({k: v for k, v in r.items() if k != 'stage_times'})}",

@@ -112,8 +112,11 @@ typedef struct pcisph_solver_options {
                                        sorted order; sums are unchanged), so a wavefront
                                        covers a compact block of cells; bit 1: workgroups are
                                        placed so that each of the 8 XCDs (own L2) works on one
-                                       contiguous part of that order.  Default 1 (the
-                                       placement measured slower on the 1M dam break). */
+                                       contiguous part of that order; bit 2: FP32 pressure
+                                       passes with 256-thread workgroups stage each
+                                       workgroup's neighbour buckets in LDS and walk them
+                                       there.  Bits 1 and 2 measured slower on the dam
+                                       breaks and are off by default.  Default 1. */
     uint32_t reserved;
 } pcisph_solver_options;
 

@@ -72,32 +72,32 @@ __device__ __forceinline__ int3 cell_of(R x, R y, R z, R cell) {
 }
 
 // The 27 buckets around cell c in the reference order: offset index
-// i = (dx+1) + 3(dy+1) + 9(dz+1), x fastest (grid_utils.cl:65-79).  The three
-// per-axis hash products are formed once and combined per bucket.
+// i = (dx+1) + 3(dy+1) + 9(dz+1), x fastest (grid_utils.cl:65-79).  The nine
+// (dz, dy) rows are a loop, not unrolled: `fn` is a whole gather, and 27 inlined
+// copies of it per walk make kernels of tens of thousands of instructions that
+// thrash the instruction cache.  The three x buckets of a row are unrolled, with
+// the x hash products formed once.
 template <typename F>
 __device__ __forceinline__ void for_each_bucket(int3 c, const HashGrid& g, F&& fn) {
-    uint32_t hx[3], hy[3], hz[3], lx[3], ly[3], lz[3];
+    uint32_t hx[3], lx[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const uint32_t x = (uint32_t)c.x + (uint32_t)(d - 1);
-        const uint32_t y = (uint32_t)c.y + (uint32_t)(d - 1);
-        const uint32_t z = (uint32_t)c.z + (uint32_t)(d - 1);
         hx[d] = x * 19349663u;
-        hy[d] = y * 73856093u;
-        hz[d] = z * 83492791u;
         lx[d] = x & 63u;
-        ly[d] = 4u * (y & 63u);
-        lz[d] = 16u * (z & 63u);
     }
+#pragma unroll 1
+    for (int row = 0; row < 9; ++row) {
+        const uint32_t y = (uint32_t)c.y + (uint32_t)(row % 3 - 1);
+        const uint32_t z = (uint32_t)c.z + (uint32_t)(row / 3 - 1);
+        const uint32_t hyz = (y * 73856093u) ^ (z * 83492791u);
+        const uint32_t lyz = 4u * (y & 63u) + 16u * (z & 63u);
 #pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const uint32_t v = ((hx[dx] ^ hy[dy] ^ hz[dz]) << 6) + lx[dx] + ly[dy] + lz[dz];
-                fn(bucket_mod(v & 0x7FFFFFFFu, g));
-            }
+        for (int dx = 0; dx < 3; ++dx) {
+            const uint32_t v = ((hx[dx] ^ hyz) << 6) + lx[dx] + lyz;
+            fn(bucket_mod(v & 0x7FFFFFFFu, g));
+        }
+    }
 }
 
 }  // namespace pcisph

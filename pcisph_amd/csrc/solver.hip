@@ -790,6 +790,7 @@ class Engine final : public EngineBase {
         auto put = [&](uint64_t v) { k.push_back(v); };
         auto ptr = [&](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
         put(L.block), put(fast), put(warm), put(features), put(digit), put(key_bits), put(maxit), put(chunk);
+        put(A_.tile_pressure);
         put(A_.n), put(A_.nb), put(A_.B);
         for (const void* p : {(const void*)A_.pos_in, (const void*)A_.vel_in, (const void*)A_.pos,
                               (const void*)A_.vel, (const void*)A_.xr, (const void*)A_.nrm, (const void*)A_.fo,
@@ -838,6 +839,7 @@ class Engine final : public EngineBase {
         const uint32_t* order = ordered ? ovals_[order_passes & 1] : nullptr;
         A_.batch = Batch<R>{scene_table_, block_table_, states_, order, S, fluid_blocks_, boundary_blocks_,
                             A_.n, A_.nb, 0};
+        A_.tile_pressure = (o.work_order & 4) != 0;
         if (features) ensure_warm_buffers(warm);
         D_.disp_next = A_.NL.on ? A_.NL.disp[1] : nullptr;
         D_.buckets = A_.NL.buckets;
@@ -1324,7 +1326,7 @@ int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "sort_digit_bits must be in 4..8");
         if (!(opt->neighbor_skin >= 0.f && opt->neighbor_skin <= 2.f))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "neighbor_skin must be in [0, 2]");
-        if (opt->work_order > 3) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..3");
+        if (opt->work_order > 7) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..7");
         s->opt = *opt;
     });
 }

@@ -851,6 +851,209 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
     if (threadIdx.x == 0 && s_list_waves) atomicAdd(&st->list_waves, s_list_waves);
 }
 
+// ------------------------------------------------------- LDS-staged pressure pass
+//
+// The fluid ranges of update_pressure (pcisph.cl:212-263) with the candidates staged
+// in LDS.  A workgroup of 256 fluid slots in the Morton work order covers a compact
+// block of ~32 cells; the union of its lanes' 27-bucket neighbourhoods is ~150
+// buckets.  The workgroup gathers those bucket ids into an LDS hash set, loads each
+// bucket's predicted positions (and boundary positions) into LDS once, and every lane
+// then walks its 27 buckets in the reference's order from LDS -- the same candidates,
+// values and summation order as the global walk, without the per-lane L2 misses.  A
+// workgroup whose neighbourhood does not fit walks global memory instead.
+// The tile stays below 64 KB of LDS: a workgroup asking for more was measured to run
+// alone on the device (each wave busy ~28 us, the launch ~64 x that).
+constexpr int kTileThreads = 256;
+constexpr uint32_t kTileSlotBits = 9;
+constexpr uint32_t kTileSlots = 1u << kTileSlotBits;  // hash-set slots
+constexpr uint32_t kTileFluid = 1792;                  // staged fluid candidates
+constexpr uint32_t kTileBoundary = 1024;
+constexpr uint32_t kTileEmpty = 0xFFFFFFFFu;
+
+struct PressureTile {
+    uint32_t key[kTileSlots];
+    uint32_t fslot[kTileSlots];  // staged fluid range of the bucket: start | count << 16
+    uint32_t bslot[kTileSlots];  // staged boundary range
+    uint32_t fsrc[kTileSlots];   // first particle of the bucket in global memory
+    uint32_t bsrc[kTileSlots];
+    float4 fx[kTileFluid];
+    float4 bx[kTileBoundary];
+    uint32_t part[2][kTileThreads / 64];
+    uint32_t overflow;
+};
+
+__device__ __forceinline__ uint32_t tile_hash(uint32_t key) { return (key * 2654435761u) >> (32 - kTileSlotBits); }
+
+__device__ __forceinline__ void tile_insert(PressureTile& T, uint32_t key) {
+    uint32_t s = tile_hash(key);
+    for (uint32_t probe = 0; probe < kTileSlots; ++probe) {
+        const uint32_t seen = T.key[s];
+        if (seen == key) return;
+        if (seen == kTileEmpty) {
+            const uint32_t prev = atomicCAS(&T.key[s], kTileEmpty, key);
+            if (prev == kTileEmpty || prev == key) return;
+        }
+        s = (s + 1) & (kTileSlots - 1);
+    }
+    T.overflow = 1u;  // the set is full
+}
+
+__device__ __forceinline__ uint32_t tile_find(const PressureTile& T, uint32_t key) {
+    uint32_t s = tile_hash(key);
+    for (uint32_t probe = 0; probe < kTileSlots && T.key[s] != key; ++probe) s = (s + 1) & (kTileSlots - 1);
+    return s;
+}
+
+// Copies `total` staged elements: element q belongs to the last slot whose staged
+// start is <= q (starts are non-decreasing over slots; an empty slot never ends a run
+// of equal starts), so every thread copies q = threadIdx.x, +256, ... independently
+// and the workgroup keeps many loads in flight.
+__device__ __forceinline__ void tile_copy(const uint32_t* __restrict__ slot, const uint32_t* __restrict__ src,
+                                          float4* __restrict__ dst, const float4* __restrict__ from,
+                                          uint32_t total) {
+    for (uint32_t q = threadIdx.x; q < total; q += kTileThreads) {
+        uint32_t lo = 0, hi = kTileSlots;  // first slot with start > q
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if ((slot[mid] & 0xFFFFu) <= q) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t s = lo - 1;
+        dst[q] = from[src[s] + (q - (slot[s] & 0xFFFFu))];
+    }
+}
+
+// Exclusive workgroup scan of two counters (fluid, boundary); totals returned.
+__device__ __forceinline__ uint2 tile_scan(PressureTile& T, uint32_t a, uint32_t b, uint2& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t xa = a, xb = b;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ya = __shfl_up(xa, off), yb = __shfl_up(xb, off);
+        if (lane >= off) xa += ya, xb += yb;
+    }
+    if (lane == 63) T.part[0][wave] = xa, T.part[1][wave] = xb;
+    __syncthreads();
+    uint32_t ba = 0, bb = 0, ta = 0, tb = 0;
+#pragma unroll
+    for (int w = 0; w < kTileThreads / 64; ++w) {
+        if (w < wave) ba += T.part[0][w], bb += T.part[1][w];
+        ta += T.part[0][w], tb += T.part[1][w];
+    }
+    total = make_uint2(ta, tb);
+    return make_uint2(ba + xa - a, bb + xb - b);
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(kTileThreads) void k_pressure_tile(
+    Batch<float> B, uint32_t iter, const uint2* __restrict__ bcells, const float4* __restrict__ bpos,
+    const uint2* __restrict__ fcells, const float4* __restrict__ xr, const float4* __restrict__ xp,
+    float* __restrict__ dv, float* __restrict__ p, float4* __restrict__ xf) {
+    using O = Ops<float, FAST>;
+    __shared__ PressureTile T;
+    __shared__ unsigned long long lds[2][8];
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<float> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
+    if (iteration_skipped<float>(st, iter, P)) return;
+    const uint2* fc = fcells + P.bucket_base;
+    const uint2* bc = bcells + P.bucket_base;
+    const uint32_t t = rg.begin + threadIdx.x;
+    const bool act = t < rg.end;
+    const uint32_t i = act ? particle_of(B, t) : 0u;
+    float4 self = make_float4(0.f, 0.f, 0.f, 0.f), pr = self;
+    int3 c = make_int3(0, 0, 0);
+    if (act) {
+        self = xr[i];
+        pr = xp[i];
+        c = cell_of4<float>(self, P.cell);
+    }
+    // 1. the workgroup's buckets
+    for (uint32_t q = threadIdx.x; q < kTileSlots; q += kTileThreads) T.key[q] = kTileEmpty;
+    if (threadIdx.x == 0) T.overflow = 0u;
+    __syncthreads();
+    if (act) for_each_bucket(c, P.grid, [&](uint32_t key) { tile_insert(T, key); });
+    __syncthreads();
+    bool staged = T.overflow == 0u;
+    // 2. staged ranges: each thread owns kTileSlots / kTileThreads consecutive slots
+    constexpr uint32_t kPer = kTileSlots / kTileThreads;
+    uint2 fr[kPer], br[kPer];
+    uint32_t fsum = 0, bsum = 0;
+    if (staged) {
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t key = T.key[threadIdx.x * kPer + q];
+            fr[q] = key != kTileEmpty ? fc[key] : make_uint2(0u, 0u);
+            br[q] = key != kTileEmpty && P.nb ? bc[key] : make_uint2(0u, 0u);
+            fsum += fr[q].y - fr[q].x;
+            bsum += br[q].y - br[q].x;
+        }
+    }
+    uint2 total;
+    const uint2 base = tile_scan(T, fsum, bsum, total);
+    staged = staged && total.x <= kTileFluid && total.y <= kTileBoundary;
+    // 3. stage the candidates
+    if (staged) {
+        uint32_t fo = base.x, bo = base.y;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t fn = fr[q].y - fr[q].x, bn = br[q].y - br[q].x;
+            T.fslot[threadIdx.x * kPer + q] = fo | (fn << 16);
+            T.bslot[threadIdx.x * kPer + q] = bo | (bn << 16);
+            T.fsrc[threadIdx.x * kPer + q] = fr[q].x;
+            T.bsrc[threadIdx.x * kPer + q] = br[q].x;
+            fo += fn;
+            bo += bn;
+        }
+    }
+    __syncthreads();
+    if (staged) {
+        tile_copy(T.fslot, T.fsrc, T.fx, xp, total.x);
+        tile_copy(T.bslot, T.bsrc, T.bx, bpos, total.y);
+    }
+    __syncthreads();
+    unsigned long long m[2] = {0, 0};  // max dv, max expansion
+    if (act) {
+        float pd = 0.f;
+        auto add = [&](const float4& o) {
+            float dx, dy, dz;
+            pd += poly6<float>(dist2<float>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+        };
+        if (staged) {
+            if (P.nb)
+                for_each_bucket(c, P.grid, [&](uint32_t key) {
+                    const uint32_t r = T.bslot[tile_find(T, key)];
+                    for (uint32_t j = r & 0xFFFFu, e = (r & 0xFFFFu) + (r >> 16); j < e; ++j) add(T.bx[j]);
+                });
+            for_each_bucket(c, P.grid, [&](uint32_t key) {
+                const uint32_t r = T.fslot[tile_find(T, key)];
+                for (uint32_t j = r & 0xFFFFu, e = (r & 0xFFFFu) + (r >> 16); j < e; ++j) add(T.fx[j]);
+            });
+        } else {
+            if (P.nb)
+                walk_buckets<float>(bc, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                                    [&](const float4& o, uint32_t) { add(o); });
+            walk_buckets<float>(fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
+                                [&](const float4& o, uint32_t) { add(o); });
+        }
+        pd *= P.density_scale;
+        const float v = fmaxf(0.f, pd - P.rho0);
+        if (self.w >= P.rho0) m[1] = Bits<float>::of(fmaxf(0.f, P.rho0 - pd));
+        dv[i] = v;
+        float pi = p[i];
+        if (v != 0.f) {
+            pi += v * P.delta;
+            p[i] = pi;
+        }
+        float4 out = self;
+        out.w = O::div(pi, self.w * self.w);
+        xf[i] = out;
+        m[0] = Bits<float>::of(v);
+    }
+    unsigned long long* const dst[2] = {&st->max_dv[iter], &st->max_exp[iter]};
+    block_max_to<2>(m, dst, lds);
+}
+
 // Publication of iteration `iter` by the first workgroup of the pressure-force launch
 // (the pressure pass that produced the results has completed): every scene that ran
 // the iteration writes its results, the number of scenes that go on is stored, and

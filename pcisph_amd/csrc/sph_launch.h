@@ -43,7 +43,8 @@ struct DeviceArrays {
     uint32_t* bkeys[2] = {nullptr, nullptr};
     uint32_t* bvals[2] = {nullptr, nullptr};
     uint2* bcells = nullptr;
-    NeighborLists NL;  // r2 == 0: no lists this step
+    NeighborLists NL;  // on == 0: no lists this step
+    bool tile_pressure = false;  // FP32 fluid pressure passes from LDS tiles (256-thread workgroups)
 };
 
 // `block` must be the workgroup size the batch's block table was built for.

@@ -2,6 +2,8 @@
 // Included once per arithmetic-mode translation unit (sph_ieee.hip, sph_fast.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "sph_launch.h"
 
 namespace pcisph {
@@ -38,6 +40,23 @@ void SphLaunch<R, FAST>::pressure(const Launch& L, const DeviceArrays<R>& A, uin
     const Batch<R>& B = A.batch;
     const uint32_t grid = B.fluid_blocks + B.boundary_blocks;
     if (!grid) return;
+    if constexpr (std::is_same<R, float>::value) {
+        if (A.tile_pressure && L.block == (uint32_t)kTileThreads) {
+            // fluid ranges from LDS tiles, boundary ranges as below
+            if (B.fluid_blocks)
+                hipLaunchKernelGGL((k_pressure_tile<FAST>), dim3(B.fluid_blocks), dim3(kTileThreads), 0, L.stream,
+                                   B, iter, A.bcells, A.bpos, A.cells, A.xr, A.xp, A.dv, A.p, A.xf);
+            if (B.boundary_blocks) {
+                Batch<R> Bb = B;
+                Bb.blocks = B.blocks + B.fluid_blocks;
+                Bb.fluid_blocks = 0;
+                hipLaunchKernelGGL((k_pressure<R, FAST>), dim3(B.boundary_blocks), dim3(L.block), 0, L.stream, Bb,
+                                   iter, A.bcells, A.bpos, A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp,
+                                   A.bxf, A.NL);
+            }
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_pressure<R, FAST>), dim3(grid), dim3(L.block), 0, L.stream, B, iter, A.bcells, A.bpos,
                        A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp, A.bxf, A.NL);
 }

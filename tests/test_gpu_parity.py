@@ -197,3 +197,19 @@ def test_list_overflow_falls_back_to_buckets(gpu, oracle_lib):
     assert so[0]["iterations"] == sg[0]["iterations"]
     for f in (Field.FLUID_DENSITIES, Field.FLUID_POSITIONS, Field.FLUID_VELOCITIES, Field.FLUID_PRESSURES):
         np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
+
+
+@pytest.mark.parametrize("work_order", [0, 2, 4, 7])
+def test_work_orders_are_exact(gpu, oracle_lib, work_order):
+    """Every work order (reference slot order, XCD placement, LDS-staged pressure
+    passes, all together) leaves the oracle's bits."""
+    scene = small_scenes()[1]
+    print(f"case work_order={work_order}", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 2, max_iterations=20)
+    sg = run(g, scene, 2, max_iterations=20, work_order=work_order)
+    assert [s["iterations"] for s in so] == [s["iterations"] for s in sg]
+    ro, rg = snapshot(o), snapshot(g)
+    for name in ro:
+        np.testing.assert_array_equal(rg[name], ro[name], err_msg=name)
