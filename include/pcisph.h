@@ -111,12 +111,12 @@ typedef struct pcisph_solver_options {
                                        of the cells (data and results stay in the reference's
                                        sorted order; sums are unchanged), so a wavefront
                                        covers a compact block of cells; bit 1: workgroups are
-                                       placed so that each of the 8 XCDs (own L2) works on one
-                                       contiguous part of that order; bit 2: FP32 pressure
+                                       placed so that each of the 8 XCDs (own L2) works on
+                                       compact chunks of that order; bit 2: FP32 pressure
                                        passes with 256-thread workgroups stage each
                                        workgroup's neighbour buckets in LDS and walk them
-                                       there.  Bits 1 and 2 measured slower on the dam
-                                       breaks and are off by default.  Default 1. */
+                                       there (measured slower on the dam breaks).
+                                       Default 3 (bits 0 and 1). */
     uint32_t reserved;
 } pcisph_solver_options;
 

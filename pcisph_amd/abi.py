@@ -181,7 +181,7 @@ class SolverBase:
 
     def set_solver_options(self, min_iterations=3, max_iterations=7, launch_boundary_init=0,
                            resort_boundary=0, block_size=0, sort_digit_bits=0, neighbor_skin=0.2,
-                           graph_iterations=4, work_order=1):
+                           graph_iterations=4, work_order=3):
         o = SolverOptions(min_iterations, max_iterations, launch_boundary_init, resort_boundary,
                           block_size, sort_digit_bits, neighbor_skin, graph_iterations, work_order)
         self._call("set_solver_options", C.byref(o))

@@ -78,7 +78,7 @@ __device__ __forceinline__ int3 cell_of(R x, R y, R z, R cell) {
 // thrash the instruction cache.  The three x buckets of a row are unrolled, with
 // the x hash products formed once.
 template <typename F>
-__device__ __forceinline__ void for_each_bucket(int3 c, const HashGrid& g, F&& fn) {
+__device__ __forceinline__ void for_each_bucket_row(int3 c, const HashGrid& g, F&& fn) {
     uint32_t hx[3], lx[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -92,12 +92,20 @@ __device__ __forceinline__ void for_each_bucket(int3 c, const HashGrid& g, F&& f
         const uint32_t z = (uint32_t)c.z + (uint32_t)(row / 3 - 1);
         const uint32_t hyz = (y * 73856093u) ^ (z * 83492791u);
         const uint32_t lyz = 4u * (y & 63u) + 16u * (z & 63u);
+        uint32_t k[3];
 #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            const uint32_t v = ((hx[dx] ^ hyz) << 6) + lx[dx] + lyz;
-            fn(bucket_mod(v & 0x7FFFFFFFu, g));
-        }
+        for (int dx = 0; dx < 3; ++dx) k[dx] = bucket_mod((((hx[dx] ^ hyz) << 6) + lx[dx] + lyz) & 0x7FFFFFFFu, g);
+        fn(k[0], k[1], k[2]);
     }
+}
+
+template <typename F>
+__device__ __forceinline__ void for_each_bucket(int3 c, const HashGrid& g, F&& fn) {
+    for_each_bucket_row(c, g, [&](uint32_t k0, uint32_t k1, uint32_t k2) {
+        fn(k0);
+        fn(k1);
+        fn(k2);
+    });
 }
 
 }  // namespace pcisph

@@ -689,9 +689,11 @@ class Engine final : public EngineBase {
     // particles in ranges of `block`, then every scene's boundary particles.
     //
     // With `xcd` the fluid ranges are placed for the 8 XCDs, to which workgroups are
-    // dispatched round-robin (workgroup w runs on XCD w mod 8): XCD x gets one
-    // contiguous run of ranges, so in the spatially coherent work order each XCD's L2
-    // holds one compact part of the domain.
+    // dispatched round-robin (workgroup w runs on XCD w mod 8): each run of 8 x 16
+    // workgroups deals 8 consecutive chunks of 16 ranges to the 8 XCDs, so an XCD's
+    // L2 sees compact parts of the spatially coherent order while the work stays
+    // interleaved over the XCDs (one contiguous eighth per XCD measured slower: the
+    // parts finish at different times).
     void prepare_blocks(uint32_t block, bool xcd) {
         const uint32_t want = block | (xcd ? 0x80000000u : 0u);
         if (blocks_for_ == want) return;
@@ -701,13 +703,12 @@ class Engine final : public EngineBase {
                 t.push_back({s, b, std::min(b + block, f_off_[s + 1]), b == f_off_[s] ? kRangeFirstOfScene : 0u});
         const uint32_t fluid_blocks = (uint32_t)t.size();
         if (xcd && fluid_blocks > 8) {
-            constexpr uint32_t kXcds = 8;
-            std::vector<BlockRange> placed(fluid_blocks);
-            uint32_t start = 0;
-            for (uint32_t x = 0; x < kXcds; ++x) {
-                const uint32_t count = x < fluid_blocks ? (fluid_blocks - 1 - x) / kXcds + 1 : 0;
-                for (uint32_t q = 0; q < count; ++q) placed[q * kXcds + x] = t[start + q];
-                start += count;
+            constexpr uint32_t kXcds = 8, kChunk = 16, kGroup = kXcds * kChunk;
+            std::vector<BlockRange> placed(t.begin(), t.end());
+            const uint32_t full = fluid_blocks / kGroup * kGroup;  // the tail keeps its order
+            for (uint32_t p = 0; p < full; ++p) {
+                const uint32_t g = p / kGroup, r = p % kGroup;
+                placed[p] = t[g * kGroup + (r % kXcds) * kChunk + r / kXcds];
             }
             std::copy(placed.begin(), placed.end(), t.begin());
         }
@@ -1156,7 +1157,7 @@ struct pcisph_solver {
     bool own_stream = false;
     std::vector<Scene> scenes = std::vector<Scene>(1);
     uint32_t current = 0;  // the scene that setters, uploads and downloads address
-    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 4, 1, 0};
+    pcisph_solver_options opt{3, 7, 0, 0, 0, 0, 0.2f, 4, 3, 0};
     int precision = kPrecisionFast;
     std::unique_ptr<EngineBase> eng;
     std::string err;

@@ -455,10 +455,12 @@ __device__ __forceinline__ float displacement(const V4<R>& a, const V4<R>& b) {
 }
 
 // Batched gathers.  A neighbour walk is split into load(j) -> data and
-// use(data, j): kGatherBatch candidates are loaded before any is used, so each lane
+// use(data, j): a batch of candidates is loaded before any is used, so each lane
 // keeps that many independent loads in flight instead of one; use() still sees the
-// candidates in order, so sums are unchanged.
+// candidates in order, so sums are unchanged.  Measured best: 4 for bucket ranges
+// (short, ~8 candidates), 8 for neighbour lists (index loads first, then data).
 constexpr int kGatherBatch = 4;
+constexpr int kListBatch = 8;
 
 template <typename Load, typename Use>
 __device__ __forceinline__ void gather_range(uint32_t start, uint32_t end, Load&& load, Use&& use) {
@@ -474,13 +476,16 @@ __device__ __forceinline__ void gather_range(uint32_t start, uint32_t end, Load&
     for (; j < end; ++j) use(load(j), j);
 }
 
-// The 27 buckets around cell c, each range gathered in batches.
+// The 27 buckets around cell c, each range gathered in batches; a row's three
+// bucket ranges are loaded before any of them is walked.
 template <typename R, typename Load, typename Use>
 __device__ __forceinline__ void walk_buckets(const uint2* __restrict__ cells, int3 c, const HashGrid& g, Load&& load,
                                              Use&& use) {
-    for_each_bucket(c, g, [&](uint32_t key) {
-        const uint2 r = cells[key];
-        gather_range(r.x, r.y, load, use);
+    for_each_bucket_row(c, g, [&](uint32_t k0, uint32_t k1, uint32_t k2) {
+        const uint2 r0 = cells[k0], r1 = cells[k1], r2 = cells[k2];
+        gather_range(r0.x, r0.y, load, use);
+        gather_range(r1.x, r1.y, load, use);
+        gather_range(r2.x, r2.y, load, use);
     });
 }
 
@@ -491,15 +496,15 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
     using T = decltype(load(0u));
     const uint32_t* q = list + i;
     uint32_t k = 0;
-    for (; k + kGatherBatch <= len; k += kGatherBatch) {
-        uint32_t jj[kGatherBatch];
-        T d[kGatherBatch];
+    for (; k + kListBatch <= len; k += kListBatch) {
+        uint32_t jj[kListBatch];
+        T d[kListBatch];
 #pragma unroll
-        for (int u = 0; u < kGatherBatch; ++u) jj[u] = q[(size_t)(k + u) * stride];
+        for (int u = 0; u < kListBatch; ++u) jj[u] = q[(size_t)(k + u) * stride];
 #pragma unroll
-        for (int u = 0; u < kGatherBatch; ++u) d[u] = load(jj[u]);
+        for (int u = 0; u < kListBatch; ++u) d[u] = load(jj[u]);
 #pragma unroll
-        for (int u = 0; u < kGatherBatch; ++u) use(d[u], jj[u]);
+        for (int u = 0; u < kListBatch; ++u) use(d[u], jj[u]);
     }
     for (; k < len; ++k) {
         const uint32_t j = q[(size_t)k * stride];
