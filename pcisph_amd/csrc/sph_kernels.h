@@ -342,14 +342,21 @@ __device__ __forceinline__ void record_displacement(uint32_t* table, uint32_t ke
     if (key != 0xFFFFFFFFu && (lane == 63 || next != key) && b) atomicMax(&table[key], b);
 }
 
-// D_i: the largest recorded displacement in the 27 buckets around cell c.
+// D_i: the largest recorded displacement in the 27 buckets around cell c (all 27
+// independent loads issued together).
 __device__ __forceinline__ float bucket_displacement(const uint32_t* __restrict__ table, int3 c,
                                                      const HashGrid& g, uint32_t base) {
+    uint32_t v[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const uint32_t x = (uint32_t)c.x + (uint32_t)(k % 3 - 1);
+        const uint32_t y = (uint32_t)c.y + (uint32_t)((k / 3) % 3 - 1);
+        const uint32_t z = (uint32_t)c.z + (uint32_t)(k / 9 - 1);
+        v[k] = table[base + hash_key(x, y, z, g)];
+    }
     uint32_t m = 0;
-    for_each_bucket(c, g, [&](uint32_t key) {
-        const uint32_t v = table[base + key];
-        m = v > m ? v : m;
-    });
+#pragma unroll
+    for (int k = 0; k < 27; ++k) m = v[k] > m ? v[k] : m;
     return __uint_as_float(m);
 }
 
@@ -780,7 +787,11 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
     const uint32_t t = rg.begin + threadIdx.x;
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
     const uint32_t i = boundary || t >= rg.end ? t : particle_of(B, t);
-    // lists for this wavefront if every lane's pairs are all in its list
+    // lists for this wavefront if every lane's pairs are all in its list; a scene
+    // fewer than 1% of whose wavefronts could use them in the step's passes so far
+    // stops testing from the third pass on (displacements hardly change within a step)
+    const uint32_t waves = (P.n + 63) / 64 + (P.nb + 63) / 64;
+    const bool try_lists = NL.on && (iter < 2 || 100ull * st->list_waves >= (unsigned long long)waves * iter);
     bool ok = true;
     V4<R> self{}, pr{};
     int3 c{};
@@ -788,12 +799,12 @@ __global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ 
         self = boundary ? bpos[i] : xr[i];
         pr = boundary ? self : xp[i];
         c = cell_of4<R>(self, P.cell);
-        if (NL.on) {
+        if (try_lists) {
             const float d = boundary ? 0.f : displacement<R>(pr, self);
             ok = d + bucket_displacement(NL.disp[iter & 1], c, P.grid, P.bucket_base) <= P.disp_limit;
         }
     }
-    const bool lists = NL.on && __all(ok);
+    const bool lists = try_lists && __all(ok);
     __syncthreads();
     if (lists && (threadIdx.x & 63) == 0) atomicAdd(&s_list_waves, 1u);
     unsigned long long m[2] = {0, 0};  // max dv, max expansion
