@@ -3,10 +3,14 @@
     python -m pcisph_amd.train_warm_start --out pcisph_amd/data/warm_start.npy
 
 Samples are (features, converged pressure change) pairs written while the solver
-runs with the previous-pressure start (the zero network), so they show what a
-warm-started loop converges to: every `stride`-th fluid particle of every step in a
-collection window, from the 1,048,576-particle dam break and from the 64-scene batch
-(counts log-uniform in [2,048, 262,144], dam breaks and sphere drops).  The network is
+runs: every `stride`-th fluid particle of every step in a collection window, from the
+1,048,576-particle dam break and from the 64-scene batch (counts log-uniform in
+[2,048, 262,144], dam breaks and sphere drops).  With `--collect plain` (default) the
+solver runs the plain loop after its start-up transient, so a target is the change
+between two converged pressures that each started from zero -- the pressure a step
+needs, whatever start reaches it (PCISPH only raises pressures within a step, so
+pressures reached from a warm start are biased towards that start).  `--collect
+previous` / `--init` collect with the previous-pressure start / a given network.  The network is
 then trained with the project's own BF16 MFMA forward/backward kernels and FP32 Adam,
 and evaluated from one saved state of the 1M dam break: mean correction iterations
 per step of the plain loop, of the previous-pressure start and of the trained network,
@@ -25,9 +29,10 @@ from . import Fluid, scenes
 from . import warm_start as ws
 
 
-def collect_from(fluid: Fluid, pre: int, steps: int, stride: int, capacity: int, batch=None) -> tuple:
-    """Run `pre` steps, then `steps` steps storing every `stride`-th particle, all with
-    the previous-pressure start."""
+def collect_from(fluid: Fluid, pre: int, steps: int, stride: int, capacity: int, batch=None,
+                 policy: np.ndarray | None = None, warm: bool = True) -> tuple:
+    """Run `pre` steps, then `steps` steps storing every `stride`-th particle, all
+    warm-started with `policy` (default: the previous-pressure start) or all plain."""
     shift = None
     if batch is not None and any(sc.meta.get("slosh") for sc in batch):
         shift = lambda k: [scenes.boundary_shift(sc, k) for sc in batch]  # noqa: E731
@@ -38,8 +43,8 @@ def collect_from(fluid: Fluid, pre: int, steps: int, stride: int, capacity: int,
                 fluid.translate_boundaries(shift(start + k))
             fluid.step()
 
-    fluid.mlp_set_params(ws.zero_params())
-    fluid.set_warm_start(True)
+    fluid.mlp_set_params(ws.zero_params() if policy is None else policy)
+    fluid.set_warm_start(warm)
     advance(pre, 0)
     fluid.samples_reserve(capacity)
     fluid.samples_collect(stride)
@@ -69,14 +74,25 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--eval-steps", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default=None,
+                    help="parameters (.npy) to collect with and to continue training from (an on-policy "
+                         "round); default: train from scratch on --collect samples")
+    ap.add_argument("--collect", choices=["plain", "previous"], default="plain",
+                    help="plain: the plain loop after the start-up transient; previous: the "
+                         "previous-pressure start from step 50")
     args = ap.parse_args(argv)
+    policy = np.load(args.init).astype(np.float32) if args.init else None
+    warm = policy is not None or args.collect == "previous"
+    # the plain loop's pressures are converged only after the 1M start-up transient
+    pre_1m, pre_batch = (400, 60) if not warm else (50, 20)
     t0 = time.time()
     half = args.samples // 2
 
     # config 1: the 1M dam break, samples over steps 50..349
     big = Fluid()
     big.load_scene(scenes.dam_break_1m(), max_iterations=50)
-    x1, t1 = collect_from(big, 50, 300, max(1, 300 * big.field_size(0) // 3 // half), half)
+    x1, t1 = collect_from(big, pre_1m, 300, max(1, 300 * big.field_size(0) // 3 // half), half, policy=policy,
+                          warm=warm)
     print(f"1M dam break: {len(t1)} samples, {time.time() - t0:.1f} s", flush=True)
 
     # config 2: the 64-scene batch, samples over steps 20..59
@@ -84,7 +100,7 @@ def main(argv=None):
     b = Fluid()
     b.load_scenes(batch, max_iterations=50)
     total = sum(sc.fluid_count for sc in batch)
-    x2, t2 = collect_from(b, 20, 40, max(1, 40 * total // half), half, batch)
+    x2, t2 = collect_from(b, pre_batch, 40, max(1, 40 * total // half), half, batch, policy=policy, warm=warm)
     print(f"64-scene batch: {len(t2)} samples, {time.time() - t0:.1f} s", flush=True)
     b.close()
 
@@ -93,8 +109,9 @@ def main(argv=None):
     state_samples = big.samples_count()
     big.samples_reserve(len(t))
     big.samples_upload(x, t)
-    losses, params = [], None
-    for lr, frac in ((3e-3, 1 / 3), (1e-3, 1 / 3), (3e-4, 1 / 3)):
+    losses, params = [], policy
+    schedule = ((3e-3, 1 / 3), (1e-3, 1 / 3), (3e-4, 1 / 3)) if policy is None else ((1e-3, 1 / 2), (3e-4, 1 / 2))
+    for lr, frac in schedule:
         losses += ws.train(big, steps=max(1, int(args.train_steps * frac)), batch=args.batch, lr=lr,
                            seed=args.seed, params=params, log_every=1000)
         params = big.mlp_get_params()
@@ -105,8 +122,10 @@ def main(argv=None):
     res = evaluate(big, params, args.eval_steps)
     meta = {"samples": int(len(t)), "samples_1m": int(len(t1)), "samples_batch": int(len(t2)),
             "collected_1m_before_cap": int(state_samples), "train_steps": args.train_steps, "batch": args.batch,
+            "collected_with": args.init or ("plain loop" if not warm else "previous-pressure start"),
+            "collection_steps_1m": [pre_1m, pre_1m + 300],
             "final_loss": losses[-1], "target_mean": float(t.mean()), "target_std": float(t.std()),
-            "evaluation_1m_from_step_350": res, "seconds": round(time.time() - t0, 1)}
+            "evaluation_1m_after_collection": res, "seconds": round(time.time() - t0, 1)}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as fh:
         json.dump(meta, fh, indent=1)
     print(json.dumps(meta), flush=True)
