@@ -151,6 +151,45 @@ def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, t
     return out
 
 
+def warm_gain(scene, weights: str, pre: int, steps: int, prec: int) -> dict:
+    """Mean correction iterations per step from one saved state (`pre` plain steps in),
+    `steps` steps each: plain loop, previous-pressure start, trained network."""
+    import pcisph_amd
+    from pcisph_amd import scenes
+    from pcisph_amd import warm_start as ws
+    f = pcisph_amd.Fluid(device=0, precision=prec)
+    f.load_scene(scene)
+    shift = (lambda k: f.translate_boundary(*scenes.boundary_shift(scene, k))) \
+        if scene.meta.get("slosh") else None  # noqa: E731
+
+    def run(n, start, warm):
+        f.set_warm_start(warm)
+        out = []
+        for k in range(n):
+            if shift:
+                shift(start + k)
+            out.append(f.step())
+        return out
+
+    run(pre, 0, False)
+    f.snapshot()
+    res = {}
+    for name, p in (("plain", None), ("previous_pressure", ws.zero_params()), ("network", np.load(weights))):
+        f.restore()
+        if p is not None:
+            f.mlp_set_params(p)
+        st = run(steps, pre, p is not None)
+        its = np.array([s["iterations"] for s in st], float)
+        res[name] = {"mean_iterations": round(float(its.mean()), 3),
+                     "max_density_error": round(float(max(s["max_density_error"] for s in st)), 5),
+                     "steps_converged": int(sum(s["converged"] for s in st))}
+    f.close()
+    res["iteration_reduction_network"] = round(1 - res["network"]["mean_iterations"] /
+                                               res["plain"]["mean_iterations"], 4)
+    res["window"] = f"steps {pre + 1}..{pre + steps} from one saved state"
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,11 +200,13 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
-    ap.add_argument("--forms", default="plain,previous_pressure,warm_start,batch_64,sloshing_plain,sloshing_warm",
+    ap.add_argument("--forms", default="plain,warm_start,batch_64,sloshing_plain,sloshing_warm,gain,gain_sloshing",
                     help="comma list of plain (reference loop), previous_pressure (each step starts from "
                          "the previous pressures), warm_start (trained network), batch_64 (64-scene batch), "
                          "sloshing_plain / sloshing_warm (the 1M block in a tank oscillating along x, "
-                         "0.5 m at 0.8 Hz; plain loop / trained network)")
+                         "0.5 m at 0.8 Hz; plain loop / trained network), gain / gain_sloshing (iterations "
+                         "per step of plain loop, previous-pressure start and network from one state 400 "
+                         "steps in, 100 steps each)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--weights", default=os.path.join(ROOT, "pcisph_amd", "data", "warm_start.npy"))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -198,6 +239,15 @@ def main() -> int:
     n = scene.fluid_count
     results = {}
     for form in forms:
+        if form in ("gain", "gain_sloshing"):
+            if not os.path.exists(args.weights):
+                continue
+            sc = scenes.sloshing_tank(0.5, 0.8, getattr(scenes, args.workload)()) \
+                if form == "gain_sloshing" else getattr(scenes, args.workload)()
+            print(f"bench: {form} {sc.name}", file=sys.stderr, flush=True)
+            results[form] = warm_gain(sc, args.weights, 400, 100, prec)
+            print(f"bench: {form} {json.dumps(results[form])}", file=sys.stderr, flush=True)
+            continue
         if form == "batch_64":
             batch = scenes.batch_64()
             total = sum(sc.fluid_count for sc in batch)
@@ -234,7 +284,11 @@ def main() -> int:
         print(f"bench: {form} {json.dumps({k: v for k, v in r.items() if k != 'stage_times'})}",
               file=sys.stderr, flush=True)
 
-    head = "plain" if "plain" in results else next(iter(results))
+    timed = [k for k, r in results.items() if "value" in r]
+    if not timed:
+        print("bench: no timed form ran", file=sys.stderr)
+        return 1
+    head = "plain" if "plain" in timed else timed[0]
     main_r = results[head]
     cpu = None
     if not args.no_cpu:
