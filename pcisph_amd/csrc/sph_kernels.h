@@ -483,16 +483,41 @@ __device__ __forceinline__ void gather_range(uint32_t start, uint32_t end, Load&
     for (; j < end; ++j) use(load(j), j);
 }
 
-// The 27 buckets around cell c, each range gathered in batches; a row's three
-// bucket ranges are loaded before any of them is walked.
-template <typename R, typename Load, typename Use>
+// The three ranges of a bucket row gathered as one sequence (r0, then r1, then r2):
+// batches of U loads run across bucket boundaries, so a row of ~24 candidates is a
+// few full batches instead of three short ranges each ending in a remainder loop
+// (measured: the pressure pass 466 -> 362 us at 1M with U = 4; the density pass
+// prefers U = 8).
+template <int U, typename Load, typename Use>
+__device__ __forceinline__ void gather_row(uint2 r0, uint2 r1, uint2 r2, Load&& load, Use&& use) {
+    using T = decltype(load(0u));
+    const uint32_t n0 = r0.y - r0.x, n01 = n0 + (r1.y - r1.x), n = n01 + (r2.y - r2.x);
+    auto at = [&](uint32_t k) { return k < n0 ? r0.x + k : (k < n01 ? r1.x + (k - n0) : r2.x + (k - n01)); };
+    uint32_t k = 0;
+    for (; k + U <= n; k += U) {
+        uint32_t jj[U];
+        T d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = at(k + u);
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[u] = load(jj[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) use(d[u], jj[u]);
+    }
+    for (; k < n; ++k) {
+        const uint32_t j = at(k);
+        use(load(j), j);
+    }
+}
+
+// The 27 buckets around cell c: a row's three bucket ranges are loaded before any of
+// them is walked, then gathered as one sequence.
+template <typename R, int U = kGatherBatch, typename Load, typename Use>
 __device__ __forceinline__ void walk_buckets(const uint2* __restrict__ cells, int3 c, const HashGrid& g, Load&& load,
                                              Use&& use) {
     for_each_bucket_row(c, g, [&](uint32_t k0, uint32_t k1, uint32_t k2) {
         const uint2 r0 = cells[k0], r1 = cells[k1], r2 = cells[k2];
-        gather_range(r0.x, r0.y, load, use);
-        gather_range(r1.x, r1.y, load, use);
-        gather_range(r2.x, r2.y, load, use);
+        gather_row<U>(r0, r1, r2, load, use);
     });
 }
 
@@ -559,7 +584,7 @@ __global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4
             uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
             ListWriter wb{NL.b, B.n_total, t, NL.cap_b, 0}, wf{NL.f, B.n_total, t, NL.cap_f, 0};
             if (P.nb)
-                walk_buckets<R>(bc, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                walk_buckets<R, kListBatch>(bc, c, P.grid, [&](uint32_t j) { return bpos[j]; },
                                 [&](const V4<R>& o, uint32_t j) {
                                     R dx, dy, dz;
                                     const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
@@ -568,7 +593,7 @@ __global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4
                                     nb += !(r2 > P.h2);
                                     if (lists && r2 <= lr2) wb.push(j);
                                 });
-            walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
+            walk_buckets<R, kListBatch>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& o, uint32_t j) {
                                 R dx, dy, dz;
                                 const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
@@ -598,7 +623,7 @@ __global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4
         if (lists) {
             const int3 c = cell_of4<R>(o, P.cell);
             ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0};
-            walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
+            walk_buckets<R, kListBatch>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& q, uint32_t j) {
                                 R dx, dy, dz;
                                 if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
