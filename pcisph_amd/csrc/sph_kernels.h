@@ -468,6 +468,7 @@ __device__ __forceinline__ float displacement(const V4<R>& a, const V4<R>& b) {
 // (short, ~8 candidates), 8 for neighbour lists (index loads first, then data).
 constexpr int kGatherBatch = 4;
 constexpr int kListBatch = 8;
+constexpr int kForceBatch = 8;  // force init: three float4 per candidate
 
 template <typename Load, typename Use>
 __device__ __forceinline__ void gather_range(uint32_t start, uint32_t end, Load&& load, Use&& use) {
@@ -521,22 +522,22 @@ __device__ __forceinline__ void walk_buckets(const uint2* __restrict__ cells, in
     });
 }
 
-// A slot-major list of `len` entries of particle i.
-template <typename Load, typename Use>
+// A slot-major list of `len` entries of particle i, U loads in flight per lane.
+template <int U = kListBatch, typename Load, typename Use>
 __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
                                           uint32_t i, Load&& load, Use&& use) {
     using T = decltype(load(0u));
     const uint32_t* q = list + i;
     uint32_t k = 0;
-    for (; k + kListBatch <= len; k += kListBatch) {
-        uint32_t jj[kListBatch];
-        T d[kListBatch];
+    for (; k + U <= len; k += U) {
+        uint32_t jj[U];
+        T d[U];
 #pragma unroll
-        for (int u = 0; u < kListBatch; ++u) jj[u] = q[(size_t)(k + u) * stride];
+        for (int u = 0; u < U; ++u) jj[u] = q[(size_t)(k + u) * stride];
 #pragma unroll
-        for (int u = 0; u < kListBatch; ++u) d[u] = load(jj[u]);
+        for (int u = 0; u < U; ++u) d[u] = load(jj[u]);
 #pragma unroll
-        for (int u = 0; u < kListBatch; ++u) use(d[u], jj[u]);
+        for (int u = 0; u < U; ++u) use(d[u], jj[u]);
     }
     for (; k < len; ++k) {
         const uint32_t j = q[(size_t)k * stride];
@@ -545,11 +546,11 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
 }
 
 // Particle i's list if it has one, else its 27 buckets (same order, same non-zero terms).
-template <typename R, typename Load, typename Use>
+template <typename R, int U = kListBatch, typename Load, typename Use>
 __device__ __forceinline__ void walk(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride, uint32_t i,
                                      const uint2* __restrict__ cells, int3 c, const HashGrid& g, Load&& load,
                                      Use&& use) {
-    if (len != kListOverflow) walk_list(list, len, stride, i, load, use);
+    if (len != kListOverflow) walk_list<U>(list, len, stride, i, load, use);
     else walk_buckets<R>(cells, c, g, load, use);
 }
 
@@ -754,7 +755,7 @@ __global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const
             }
         };
         const uint32_t len = NL.on ? NL.fcount[t] : kListOverflow;
-        walk<R>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, load, body);
+        walk<R, kForceBatch>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, load, body);
         const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
         const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
         const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
