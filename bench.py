@@ -114,16 +114,21 @@ def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, t
     elif scene is not None and scene.meta.get("slosh"):
         shift = lambda k: fluid.translate_boundary(*scenes.boundary_shift(scene, k))  # noqa: E731
 
+    lat = []
+
     def go(count, start):
         out = []
         for k in range(count):
+            t = time.perf_counter()
             if shift:
                 shift(start + k)
             out.append(fluid.step_scenes() if batch is not None else [fluid.step()])
+            lat.append(time.perf_counter() - t)
         return out
 
     go(warmup, 0)
     fluid.sync()
+    lat.clear()
     if profile:
         fluid.profile_enable(True)
     t0 = time.perf_counter()
@@ -146,6 +151,8 @@ def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, t
            "mean_fluid_candidates": round(cf, 2), "mean_boundary_candidates": round(cb, 2)}
     if batch is not None:
         out["mean_iterations_slowest_scene"] = round(float(its.max(axis=1).mean()), 3)
+        # every scene of a batch completes its step when the batch's step returns
+        out["step_latency_ms_p50_p99"] = [round(1e3 * float(np.percentile(lat, q)), 3) for q in (50, 99)]
     if times is not None:
         out["stage_times"], out["roofline"] = roofline_of(times, n, cf, cb, traffic)
     return out
@@ -200,13 +207,15 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
-    ap.add_argument("--forms", default="plain,warm_start,batch_64,sloshing_plain,sloshing_warm,gain,gain_sloshing",
+    ap.add_argument("--forms", default="plain,warm_start,batch_64,batch_256,batch_256_warm,sloshing_plain,"
+                                       "sloshing_warm,gain,gain_sloshing",
                     help="comma list of plain (reference loop), previous_pressure (each step starts from "
                          "the previous pressures), warm_start (trained network), batch_64 (64-scene batch), "
                          "sloshing_plain / sloshing_warm (the 1M block in a tank oscillating along x, "
                          "0.5 m at 0.8 Hz; plain loop / trained network), gain / gain_sloshing (iterations "
                          "per step of plain loop, previous-pressure start and network from one state 400 "
-                         "steps in, 100 steps each)")
+                         "steps in, 100 steps each), batch_256 / batch_256_warm (256 scenes of 1,024..65,536 "
+                         "particles: dam breaks, sphere drops, sloshing tanks; plain / trained network)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--weights", default=os.path.join(ROOT, "pcisph_amd", "data", "warm_start.npy"))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -248,12 +257,17 @@ def main() -> int:
             results[form] = warm_gain(sc, args.weights, 400, 100, prec)
             print(f"bench: {form} {json.dumps(results[form])}", file=sys.stderr, flush=True)
             continue
-        if form == "batch_64":
-            batch = scenes.batch_64()
+        if form in ("batch_64", "batch_256", "batch_256_warm"):
+            batch = scenes.batch_64() if form == "batch_64" else scenes.batch_256()
             total = sum(sc.fluid_count for sc in batch)
             fluid = pcisph_amd.Fluid(device=0, precision=prec)
             fluid.load_scenes(batch)
-            print(f"bench: batch_64 {len(batch)} scenes N={total} steps={args.batch_steps}", file=sys.stderr,
+            if form == "batch_256_warm":
+                if not os.path.exists(args.weights):
+                    continue
+                fluid.mlp_set_params(np.load(args.weights))
+                fluid.set_warm_start(True)
+            print(f"bench: {form} {len(batch)} scenes N={total} steps={args.batch_steps}", file=sys.stderr,
                   flush=True)
             r = measure(fluid, args.batch_steps, args.warmup, total, not args.no_profile, batch=batch)
             # scenes whose loop has ended skip the remaining iterations, so the per-particle
@@ -289,6 +303,7 @@ def main() -> int:
         print("bench: no timed form ran", file=sys.stderr)
         return 1
     head = "plain" if "plain" in timed else timed[0]
+    big = ("batch_64", "batch_256", "batch_256_warm")
     main_r = results[head]
     cpu = None
     if not args.no_cpu:
@@ -297,10 +312,10 @@ def main() -> int:
 
     line = {
         "metric": METRIC, "value": main_r["value"], "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps if head != "batch_64" else args.batch_steps, "warmup": args.warmup,
+        "steps": args.steps if head not in big else args.batch_steps, "warmup": args.warmup,
         "ms_per_step": main_r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32", "data": "synthetic",
-        "config": {"workload": scene.name if head != "batch_64" else "batch_64", "form": head,
+        "config": {"workload": scene.name if head not in big else head, "form": head,
                    "fluid_particles": n, "boundary_particles": scene.boundary_count,
                    "precision": args.precision, "density_error_bound": 0.01,
                    "iterations_min_max": [3, 50],
