@@ -386,6 +386,12 @@ __device__ __forceinline__ T block_reduce_sum(T v, T* lds) {
 
 // ------------------------------------------------------------- kernels
 //
+// The step kernels run workgroups of at most kMaxBlockThreads (the solver's
+// block_size option), which lets the compiler give a lane up to 256 VGPRs: the force
+// initialisation's batched gathers (three float4 per candidate) spill to scratch under
+// the 128 a 1024-thread bound allows.
+constexpr int kMaxBlockThreads = 512;
+//
 // Every step kernel starts from its workgroup's BlockRange: the scene (whose
 // SphParams and LoopState it uses) and the range of particle indices it covers.
 // Indices are batch-wide; a scene's buckets are its own slice of the cell tables.
@@ -560,7 +566,7 @@ __device__ __forceinline__ void walk(const uint32_t* __restrict__ list, uint32_t
 // (boundary_pressure_initialization, pcisph.cl:114-119) and record their fluid
 // neighbours.
 template <typename R, bool FAST>
-__global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+__global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
                           const uint2* __restrict__ fcells, const V4<R>* __restrict__ pos,
                           V4<R>* __restrict__ xr, uint4* __restrict__ counts, R* __restrict__ bp,
                           V4<R>* __restrict__ bxf, NeighborLists NL) {
@@ -643,7 +649,7 @@ __global__ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4
 
 // update_normal (pcisph.cl:94-112); fluid ranges only
 template <typename R, bool FAST>
-__global__ void k_normal(Batch<R> B, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+__global__ __launch_bounds__(kMaxBlockThreads) void k_normal(Batch<R> B, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                          V4<R>* __restrict__ nrm, NeighborLists NL) {
     using O = Ops<R, FAST>;
     const BlockRange rg = B.blocks[blockIdx.x];
@@ -694,7 +700,7 @@ __device__ __forceinline__ void integrate(const SphParams<R>& P, const V4<R>& x,
 // With features requested it also gathers the warm-start network's inputs; the
 // velocity's w component carries the previous step's converged pressure.
 template <typename R, bool FAST>
-__global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+__global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                              const V4<R>* __restrict__ vel, const V4<R>* __restrict__ nrm,
                              V4<R>* __restrict__ fo, R* __restrict__ p, V4<R>* __restrict__ fp,
                              V4<R>* __restrict__ xp, FeatureArgs F, NeighborLists NL) {
@@ -795,7 +801,7 @@ __global__ void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const
 // lists (built at x with radius (1+skin)h) only where every lane's displacement test
 // passes (NeighborLists), else its 27 buckets.
 template <typename R, bool FAST>
-__global__ void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
                            const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
                            const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                            const V4<R>* __restrict__ xp, R* __restrict__ dv, R* __restrict__ p,
@@ -1144,7 +1150,7 @@ __device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* 
 // ends the scene's loop, the final in-place integration (Fluid.cpp:309-316) into
 // pos_out/vel_out.  Fluid ranges only.
 template <typename R, bool FAST>
-__global__ void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
                                  const V4<R>* __restrict__ bxf, const uint2* __restrict__ fcells,
                                  const V4<R>* __restrict__ xf, const V4<R>* __restrict__ fo,
                                  const V4<R>* __restrict__ vel, V4<R>* __restrict__ fp, V4<R>* __restrict__ xp,
@@ -1244,7 +1250,7 @@ __global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, const V4<
 // smaller residual max |rho* - rho0| (k_select_start), so a prediction that makes
 // the residual worse is dropped and the loop is exactly the plain one.
 template <typename R, bool FAST>
-__global__ void k_pressure_dual(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
                                 const R* __restrict__ binit, const uint2* __restrict__ fcells,
                                 const V4<R>* __restrict__ xr, const V4<R>* __restrict__ xp, R* __restrict__ dv,
                                 R* __restrict__ p, V4<R>* __restrict__ xf, R* __restrict__ bp,
@@ -1379,7 +1385,7 @@ __global__ void k_adopt_plain_start(Batch<R> B, DualArrays<R> D, V4<R>* __restri
 // initialize_boundary_boundary_pred_densities (pcisph.cl:195-210), opt-in; `blocks`
 // is the boundary part of the block table.
 template <typename R, bool FAST>
-__global__ void k_boundary_init(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
+__global__ __launch_bounds__(kMaxBlockThreads) void k_boundary_init(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
                                 const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
                                 R* __restrict__ binit) {
     const BlockRange rg = blocks[blockIdx.x];
