@@ -99,6 +99,55 @@ __device__ __forceinline__ void for_each_bucket_row(int3 c, const HashGrid& g, F
     }
 }
 
+// Where a scene's bucket ranges are read.  `cells` is the scene's slice of the
+// bucket table (indexed by hash key).  `grid`, when dim[0] != 0, is a dense copy for
+// the cells of a box [lo, lo + dim): entry rx + dim[0] (ry + dim[1] rz) holds the
+// range of the bucket that cell lo + (rx, ry, rz) hashes to, so a neighbourhood's
+// lookups read neighbouring words instead of hash-scattered ones.  Both give the
+// same range for a cell; cells outside the box use the bucket table.
+struct CellTable {
+    const uint2* cells;
+    const uint2* grid;
+    HashGrid g;
+    uint32_t lo[3];   // box origin (cell coordinates as 32-bit words)
+    uint32_t dim[3];  // box size in cells; dim[0] == 0: no box
+};
+
+// The bucket ranges of the 27 cells around c, one (dz, dy) row of three x-adjacent
+// cells at a time, in the reference's order (grid_utils.cl:65-79): fn(r0, r1, r2).
+// A row inside the box reads three consecutive grid entries; any other row hashes.
+template <typename F>
+__device__ __forceinline__ void for_each_range_row(const CellTable& T, int3 c, F&& fn) {
+    const uint32_t x0 = (uint32_t)c.x - 1u;
+    const uint32_t rx = x0 - T.lo[0];
+    const bool xin = T.dim[0] >= 3u && rx <= T.dim[0] - 3u;
+    uint32_t hx[3], lx[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        hx[d] = (x0 + (uint32_t)d) * 19349663u;
+        lx[d] = (x0 + (uint32_t)d) & 63u;
+    }
+#pragma unroll 1
+    for (int row = 0; row < 9; ++row) {
+        const uint32_t y = (uint32_t)c.y + (uint32_t)(row % 3 - 1);
+        const uint32_t z = (uint32_t)c.z + (uint32_t)(row / 3 - 1);
+        const uint32_t ry = y - T.lo[1], rz = z - T.lo[2];
+        uint2 r[3];
+        if (xin && ry < T.dim[1] && rz < T.dim[2]) {
+            const uint2* q = T.grid + (rx + T.dim[0] * (ry + T.dim[1] * rz));
+#pragma unroll
+            for (int d = 0; d < 3; ++d) r[d] = q[d];
+        } else {
+            const uint32_t hyz = (y * 73856093u) ^ (z * 83492791u);
+            const uint32_t lyz = 4u * (y & 63u) + 16u * (z & 63u);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                r[d] = T.cells[bucket_mod((((hx[d] ^ hyz) << 6) + lx[d] + lyz) & 0x7FFFFFFFu, T.g)];
+        }
+        fn(r[0], r[1], r[2]);
+    }
+}
+
 template <typename F>
 __device__ __forceinline__ void for_each_bucket(int3 c, const HashGrid& g, F&& fn) {
     for_each_bucket_row(c, g, [&](uint32_t k0, uint32_t k1, uint32_t k2) {

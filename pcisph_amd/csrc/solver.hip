@@ -389,6 +389,9 @@ class Engine final : public EngineBase {
     void free_internal() {
         dfree(A_.cells);
         dfree(A_.bcells);
+        dfree(A_.fgrid);
+        dfree(A_.bgrid);
+        dfree(ext_dev_);
         dfree(sort_scratch_);
     }
 
@@ -465,6 +468,7 @@ class Engine final : public EngineBase {
             states_n_ = (uint32_t)n.size();
         }
         blocks_for_ = 0;  // block table out of date
+        boxes_dirty_ = true;
         HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
@@ -481,6 +485,7 @@ class Engine final : public EngineBase {
         }
         buckets_ = buckets;
         bucket_base_ = base;
+        boxes_dirty_ = true;
         const size_t words = std::max(radix_sort_scratch_words(A_.n, 8), radix_sort_scratch_words(A_.nb, 8));
         if (words > sort_scratch_words_) {
             dalloc(sort_scratch_, words, stream_);
@@ -542,6 +547,7 @@ class Engine final : public EngineBase {
         HIP_CHECK(hipMemcpyAsync(A_.vel_in, snap_vel_, sizeof(V4<R>) * A_.n, hipMemcpyDeviceToDevice, stream_));
         HIP_CHECK(hipMemcpyAsync(A_.bpos, snap_b_, sizeof(V4<R>) * A_.nb, hipMemcpyDeviceToDevice, stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
+        boxes_dirty_ = true;
     }
 
     void drop_snapshot() {
@@ -612,6 +618,7 @@ class Engine final : public EngineBase {
         }
         if (v.vec) DataLaunch<R>::unpack_vec3(L, from, v.vec, v.n);
         else DataLaunch<R>::unpack_scalar(L, from, v.scalar, v.n, v.stride, v.offset);
+        if (field == PCISPH_FLUID_POSITIONS || field == PCISPH_BOUNDARY_POSITIONS) boxes_dirty_ = true;
         HIP_CHECK(hipGetLastError());
         if (!device) HIP_CHECK(hipStreamSynchronize(stream_));  // staging reuse + host buffer lifetime
     }
@@ -741,6 +748,13 @@ class Engine final : public EngineBase {
             const float h = (float)P.h, radius = (1.f + list_skin_) * h;
             P.list_r2 = list_skin_ > 0.f ? radius * radius : 0.f;
             P.disp_limit = 0.9f * list_skin_ * h;
+            if (s < boxes_.size()) {
+                for (int a = 0; a < 3; ++a) {
+                    P.box_lo[a] = boxes_[s].lo[a];
+                    P.box_dim[a] = boxes_[s].dim[a];
+                }
+                P.box_base = boxes_[s].base;
+            }
             t[s] = P;
         }
         if (t.size() == scene_host_.size() &&
@@ -752,6 +766,84 @@ class Engine final : public EngineBase {
                                  stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
         scene_host_ = t;
+    }
+
+    // The cell boxes of the dense range grids (CellTable): each scene's box covers the
+    // cells of its fluid and boundary particles with a margin of two cells -- the
+    // boundary's alone when that is larger than kBoxCellsPerParticle cells per
+    // particle, none when that is too.  Measured once after positions are set (upload,
+    // restore, new counts); particles that later leave the box look their
+    // neighbourhoods up in the bucket tables.  Needs the block and scene tables.
+    static constexpr uint64_t kBoxCellsPerParticle = 8;
+    void rebuild_boxes(const Launch& L, size_t S) {
+        std::vector<int> ext(12 * S);
+        for (size_t s = 0; s < S; ++s)
+            for (int a = 0; a < 3; ++a) {
+                ext[12 * s + a] = ext[12 * s + 6 + a] = INT32_MAX;
+                ext[12 * s + 3 + a] = ext[12 * s + 9 + a] = INT32_MIN;
+            }
+        if (ext_cap_ < ext.size()) {
+            dalloc(ext_dev_, ext.size(), stream_);
+            ext_cap_ = ext.size();
+        }
+        if (S) {
+            HIP_CHECK(hipMemcpyAsync(ext_dev_, ext.data(), sizeof(int) * ext.size(), hipMemcpyHostToDevice, stream_));
+            Batch<R> B{};
+            B.scenes = scene_table_;
+            B.blocks = block_table_;
+            B.fluid_blocks = fluid_blocks_;
+            B.boundary_blocks = boundary_blocks_;
+            DataLaunch<R>::extents(L, B, A_.pos_in, A_.bpos, ext_dev_);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(ext.data(), ext_dev_, sizeof(int) * ext.size(), hipMemcpyDeviceToHost, stream_));
+            HIP_CHECK(hipStreamSynchronize(stream_));
+        }
+        boxes_.assign(S, SceneBox{});
+        uint64_t total = 0;
+        uint32_t largest = 0;
+        for (size_t s = 0; s < S; ++s) {
+            const uint64_t limit = kBoxCellsPerParticle * ((uint64_t)n_[s] + nb_[s]) + 4096;
+            auto fit = [&](bool fluid, SceneBox& b) {
+                int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+                bool any = false;
+                for (int part = fluid ? 0 : 1; part < 2; ++part) {
+                    if ((part == 0 ? n_[s] : nb_[s]) == 0) continue;
+                    any = true;
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = std::min<int64_t>(lo[a], ext[12 * s + 6 * part + a]);
+                        hi[a] = std::max<int64_t>(hi[a], ext[12 * s + 6 * part + 3 + a]);
+                    }
+                }
+                if (!any) return false;
+                uint64_t cells = 1;
+                for (int a = 0; a < 3; ++a) {
+                    const uint64_t d = (uint64_t)(hi[a] - lo[a] + 5);
+                    cells *= d;
+                    if (cells > limit) return false;
+                    b.lo[a] = (uint32_t)(lo[a] - 2);  // cell coordinates as 32-bit words
+                    b.dim[a] = (uint32_t)d;
+                }
+                return total + cells <= 0xFFFFFFFFull;
+            };
+            SceneBox b{};
+            if (fit(true, b) || fit(false, b)) {
+                const uint64_t cells = (uint64_t)b.dim[0] * b.dim[1] * b.dim[2];
+                b.base = (uint32_t)total;
+                total += cells;
+                largest = std::max(largest, (uint32_t)cells);
+                boxes_[s] = b;
+            }
+        }
+        if (total > box_cap_) {
+            HIP_CHECK(hipStreamSynchronize(stream_));
+            dalloc(A_.fgrid, total, stream_);
+            dalloc(A_.bgrid, total, stream_);
+            box_cap_ = total;
+        }
+        A_.box_cells = (uint32_t)total;
+        box_max_ = largest;
+        boxes_dirty_ = false;
+        bgrid_stale_ = true;
     }
 
     // Captures the launches `fn` makes on the stream into an executable graph.
@@ -811,6 +903,7 @@ class Engine final : public EngineBase {
         const NeighborLists& NL = A_.NL;
         ptr(NL.f), ptr(NL.b), ptr(NL.bf), ptr(NL.fcount), ptr(NL.bcount), ptr(NL.bfcount);
         ptr(NL.disp[0]), ptr(NL.disp[1]), ptr(D_.disp_next);
+        ptr(B.fgrid), ptr(B.bgrid), put(A_.box_cells), put(box_max_);
         put(NL.cap_f), put(NL.cap_b), put(NL.cap_bf), put(NL.on), put(NL.buckets), put(D_.buckets);
         return k;
     }
@@ -830,6 +923,10 @@ class Engine final : public EngineBase {
         prepare_blocks(L.block, (o.work_order & 2) != 0);
         setup_lists(o.neighbor_skin);
         prepare_scenes(scenes, o);
+        if (boxes_dirty_ || boxes_.size() != S) {
+            rebuild_boxes(L, S);
+            prepare_scenes(scenes, o);
+        }
         // work order: Morton codes with `obits` bits per axis below the scene index
         const uint32_t sbits = key_bits_for(S);
         const uint32_t obits = std::min(7u, sbits < 32 ? (32u - sbits) / 3u : 0u);
@@ -839,7 +936,8 @@ class Engine final : public EngineBase {
         const uint32_t order_passes = (order_bits + digit - 1) / digit;
         const uint32_t* order = ordered ? ovals_[order_passes & 1] : nullptr;
         A_.batch = Batch<R>{scene_table_, block_table_, states_, order, S, fluid_blocks_, boundary_blocks_,
-                            A_.n, A_.nb, 0};
+                            A_.n, A_.nb, 0, A_.box_cells ? A_.fgrid : nullptr,
+                            A_.box_cells ? A_.bgrid : nullptr};
         A_.tile_pressure = (o.work_order & 4) != 0;
         if (features) ensure_warm_buffers(warm);
         D_.disp_next = A_.NL.on ? A_.NL.disp[1] : nullptr;
@@ -872,6 +970,11 @@ class Engine final : public EngineBase {
                 if (o.launch_boundary_init) launch(fast).boundary_init(L, A_);
             });
         }
+        if (A_.box_cells && A_.nb && (bgrid_stale_ || sort_boundary)) {
+            DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.bcells, A_.bgrid);
+            HIP_CHECK(hipGetLastError());
+            bgrid_stale_ = false;
+        }
         const LaunchTable ops = launch(fast);
         // everything before the correction loop
         auto prologue = [&] {
@@ -892,6 +995,7 @@ class Engine final : public EngineBase {
                     if (res != (int)(order_passes & 1))
                         throw Error(PCISPH_ERR_INTERNAL, "work-order sort ended in an unexpected buffer");
                 }
+                if (A_.box_cells) DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.cells, A_.fgrid);
             });
             prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
             prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, A_); });
@@ -1085,6 +1189,15 @@ class Engine final : public EngineBase {
     volatile SceneResult* results_ = nullptr;
     SceneResult* dev_results_ = nullptr;
     int fluid_sorted_ = 0, boundary_sorted_ = 0;
+    struct SceneBox {
+        uint32_t lo[3], dim[3], base;
+    };
+    std::vector<SceneBox> boxes_;           // per scene; dim[0] == 0: no box
+    bool boxes_dirty_ = true, bgrid_stale_ = true;
+    int* ext_dev_ = nullptr;
+    size_t ext_cap_ = 0;
+    uint64_t box_cap_ = 0;                  // grid entries allocated
+    uint32_t box_max_ = 0;                  // cells of the largest box
     uint32_t* sort_scratch_ = nullptr;
     size_t sort_scratch_words_ = 0;
     float* staging_ = nullptr;

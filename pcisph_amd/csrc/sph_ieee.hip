@@ -1,6 +1,8 @@
 // sph_ieee.hip - IEEE instantiations of the step kernels (built with
 // -ffp-contract=off): FP32 in the reference's order of operations, and FP64.
 // Also the precision-only data kernels (hash, reorder, layout conversion).
+#include <algorithm>
+
 #include "sph_launch_impl.h"
 
 namespace pcisph {
@@ -109,6 +111,21 @@ void DataLaunch<R>::translate_scenes(const Launch& L, const Batch<R>& B, const f
     if (!B.boundary_blocks) return;
     hipLaunchKernelGGL((k_translate_scenes<R>), dim3(B.boundary_blocks), dim3(L.block), 0, L.stream,
                        B.blocks + B.fluid_blocks, shift, bpos);
+}
+
+template <typename R>
+void DataLaunch<R>::fill_box(const Launch& L, const SphParams<R>* scenes, uint32_t scene_count, uint32_t max_cells,
+                             const uint2* cells, uint2* grid) {
+    if (!scene_count || !max_cells) return;
+    const uint32_t blocks = std::min(grid_for(max_cells, 256), 1024u);
+    hipLaunchKernelGGL((k_fill_box<R>), dim3(blocks, scene_count), dim3(256), 0, L.stream, scenes, cells, grid);
+}
+
+template <typename R>
+void DataLaunch<R>::extents(const Launch& L, const Batch<R>& B, const V4<R>* pos, const V4<R>* bpos, int* ext) {
+    const uint32_t grid = B.fluid_blocks + B.boundary_blocks;
+    if (!grid) return;
+    hipLaunchKernelGGL((k_extents<R>), dim3(grid), dim3(L.block), 0, L.stream, B.scenes, B.blocks, pos, bpos, ext);
 }
 
 template <typename R>

@@ -62,6 +62,9 @@ struct SphParams {
     float div_scale;                 // m * 45 / (pi h^6) * dt * 100: warm-start divergence unit
     float list_r2;                   // neighbour-list radius squared, ((1 + skin) h)^2
     float disp_limit;                // 0.9 skin h: the largest d_i + D_i with which particle i may use its lists
+    uint32_t box_lo[3];              // cell box of the dense range grids (CellTable); box_dim[0] == 0: none
+    uint32_t box_dim[3];
+    uint32_t box_base;               // its first entry in the batch's grids
 };
 
 // One workgroup's share of a launch: particles [begin, end) of one scene.
@@ -156,7 +159,24 @@ struct Batch {
     uint32_t fluid_blocks, boundary_blocks;
     uint32_t n_total, nb_total;   // list strides
     uint32_t pad;
+    const uint2* fgrid;           // dense range grids of the scenes' cell boxes (fluid, boundary), or null
+    const uint2* bgrid;
 };
+
+// A scene's fluid (or boundary) bucket ranges: the batch's bucket table and grid.
+template <typename R>
+__device__ __forceinline__ CellTable cell_table(const SphParams<R>& P, const uint2* cells, const uint2* grid) {
+    CellTable T;
+    T.cells = cells + P.bucket_base;
+    T.grid = grid ? grid + P.box_base : nullptr;
+    T.g = P.grid;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        T.lo[a] = P.box_lo[a];
+        T.dim[a] = grid ? P.box_dim[a] : 0u;
+    }
+    return T;
+}
 
 template <typename R>
 __device__ __forceinline__ uint32_t particle_of(const Batch<R>& B, uint32_t t) {
@@ -427,6 +447,54 @@ __global__ void k_reorder(uint32_t n, const uint32_t* __restrict__ keys, const u
     if (d == n - 1 || k != keys[d + 1]) cells[k].y = d + 1;
 }
 
+// The dense range grid of scene blockIdx.y's cell box (CellTable): each box cell's
+// bucket range, copied from the bucket table after a sort.
+template <typename R>
+__global__ void k_fill_box(const SphParams<R>* __restrict__ scenes, const uint2* __restrict__ cells,
+                           uint2* __restrict__ grid) {
+    const SphParams<R>& P = scenes[blockIdx.y];
+    const uint32_t dx = P.box_dim[0], dy = P.box_dim[1], total = dx * dy * P.box_dim[2];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        const uint32_t rx = q % dx, rest = q / dx, ry = rest % dy, rz = rest / dy;
+        const uint32_t k = hash_key(P.box_lo[0] + rx, P.box_lo[1] + ry, P.box_lo[2] + rz, P.grid);
+        grid[P.box_base + q] = cells[P.bucket_base + k];
+    }
+}
+
+// Cell extents of every scene's fluid and boundary particles: ext[12 s ..] holds the
+// fluid minimum (x, y, z) and maximum, then the boundary minimum and maximum, as
+// int32 folded with atomics (the caller sets minima to INT32_MAX, maxima to INT32_MIN).
+template <typename R>
+__global__ void k_extents(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
+                          const V4<R>* __restrict__ pos, const V4<R>* __restrict__ bpos, int* __restrict__ ext) {
+    const BlockRange rg = blocks[blockIdx.x];
+    const uint32_t i = rg.begin + threadIdx.x;
+    const bool boundary = (rg.flags & kRangeBoundary) != 0;
+    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    if (i < rg.end) {
+        const int3 c = cell_of4<R>(boundary ? bpos[i] : pos[i], scenes[rg.scene].cell);
+        lo[0] = hi[0] = c.x;
+        lo[1] = hi[1] = c.y;
+        lo[2] = hi[2] = c.z;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], off));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], off));
+        }
+    }
+    if ((threadIdx.x & 63) == 0 && rg.begin + (threadIdx.x & ~63u) < rg.end) {
+        int* e = ext + 12 * rg.scene + (boundary ? 6 : 0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(e + a, lo[a]);
+            atomicMax(e + 3 + a, hi[a]);
+        }
+    }
+}
+
 // Wave-wide maximum of a non-negative float, folded into *dst (uint bits order =
 // float order for x >= 0).  Every lane of the wave must call it.
 __device__ __forceinline__ void wave_max_to(unsigned int* dst, float v) {
@@ -520,12 +588,8 @@ __device__ __forceinline__ void gather_row(uint2 r0, uint2 r1, uint2 r2, Load&& 
 // The 27 buckets around cell c: a row's three bucket ranges are loaded before any of
 // them is walked, then gathered as one sequence.
 template <typename R, int U = kGatherBatch, typename Load, typename Use>
-__device__ __forceinline__ void walk_buckets(const uint2* __restrict__ cells, int3 c, const HashGrid& g, Load&& load,
-                                             Use&& use) {
-    for_each_bucket_row(c, g, [&](uint32_t k0, uint32_t k1, uint32_t k2) {
-        const uint2 r0 = cells[k0], r1 = cells[k1], r2 = cells[k2];
-        gather_row<U>(r0, r1, r2, load, use);
-    });
+__device__ __forceinline__ void walk_buckets(const CellTable& T, int3 c, Load&& load, Use&& use) {
+    for_each_range_row(T, c, [&](uint2 r0, uint2 r1, uint2 r2) { gather_row<U>(r0, r1, r2, load, use); });
 }
 
 // A slot-major list of `len` entries of particle i, U loads in flight per lane.
@@ -554,10 +618,9 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
 // Particle i's list if it has one, else its 27 buckets (same order, same non-zero terms).
 template <typename R, int U = kListBatch, typename Load, typename Use>
 __device__ __forceinline__ void walk(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride, uint32_t i,
-                                     const uint2* __restrict__ cells, int3 c, const HashGrid& g, Load&& load,
-                                     Use&& use) {
+                                     const CellTable& T, int3 c, Load&& load, Use&& use) {
     if (len != kListOverflow) walk_list<U>(list, len, stride, i, load, use);
-    else walk_buckets<R>(cells, c, g, load, use);
+    else walk_buckets<R>(T, c, load, use);
 }
 
 // update_density (pcisph.cl:65-92), plus per-particle candidate / neighbour counts,
@@ -574,7 +637,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
-    const uint2* fc = fcells + P.bucket_base;
+    const CellTable FT = cell_table(P, fcells, B.fgrid);
     const uint32_t t = rg.begin + threadIdx.x;
     const bool in = t < rg.end;
     const bool lists = NL.on != 0;
@@ -584,14 +647,13 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
     if (!(rg.flags & kRangeBoundary)) {
         if (in) {
             const uint32_t i = particle_of(B, t);
-            const uint2* bc = bcells + P.bucket_base;
             const V4<R> self = pos[i];
             const int3 c = cell_of4<R>(self, P.cell);
             R rho = 0;
             uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
             ListWriter wb{NL.b, B.n_total, t, NL.cap_b, 0}, wf{NL.f, B.n_total, t, NL.cap_f, 0};
             if (P.nb)
-                walk_buckets<R, kListBatch>(bc, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                walk_buckets<R, kListBatch>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
                                 [&](const V4<R>& o, uint32_t j) {
                                     R dx, dy, dz;
                                     const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
@@ -600,7 +662,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
                                     nb += !(r2 > P.h2);
                                     if (lists && r2 <= lr2) wb.push(j);
                                 });
-            walk_buckets<R, kListBatch>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
+            walk_buckets<R, kListBatch>(FT, c, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& o, uint32_t j) {
                                 R dx, dy, dz;
                                 const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
@@ -630,7 +692,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
         if (lists) {
             const int3 c = cell_of4<R>(o, P.cell);
             ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0};
-            walk_buckets<R, kListBatch>(fc, c, P.grid, [&](uint32_t j) { return pos[j]; },
+            walk_buckets<R, kListBatch>(FT, c, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& q, uint32_t j) {
                                 R dx, dy, dz;
                                 if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
@@ -670,7 +732,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_normal(Batch<R> B, const u
         nz += O::div(w * dz, o.w);
     };
     const uint32_t len = NL.on ? NL.fcount[t] : kListOverflow;
-    walk<R>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xr[j]; }, body);
+    walk<R>(NL.f, len, B.n_total, t, cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xr[j]; }, body);
     V4<R> out;
     out.x = nx * P.normal_scale;
     out.y = ny * P.normal_scale;
@@ -761,7 +823,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, con
             }
         };
         const uint32_t len = NL.on ? NL.fcount[t] : kListOverflow;
-        walk<R, kForceBatch>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, load, body);
+        walk<R, kForceBatch>(NL.f, len, B.n_total, t, cell_table(P, fcells, B.fgrid), c, load, body);
         const R ox = ((R(0) + vx * P.visc_scale) + (cx * P.coh_scale + ux * P.curv_scale));
         const R oy = ((P.gravity_force + vy * P.visc_scale) + (cy * P.coh_scale + uy * P.curv_scale));
         const R oz = ((R(0) + vz * P.visc_scale) + (cz * P.coh_scale + uz * P.curv_scale));
@@ -815,7 +877,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
     LoopState* st = B.states + rg.scene;
     if (iteration_skipped<R>(st, iter, P)) return;
     if (threadIdx.x == 0) s_list_waves = 0;
-    const uint2* fc = fcells + P.bucket_base;
+    const CellTable FT = cell_table(P, fcells, B.fgrid);
     const uint32_t t = rg.begin + threadIdx.x;
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
     const uint32_t i = boundary || t >= rg.end ? t : particle_of(B, t);
@@ -845,7 +907,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
             R pd = 0;
             if (P.nb) {
                 const uint32_t len = lists ? NL.bcount[t] : kListOverflow;
-                walk<R>(NL.b, len, B.n_total, t, bcells + P.bucket_base, c, P.grid,
+                walk<R>(NL.b, len, B.n_total, t, cell_table(P, bcells, B.bgrid), c,
                         [&](uint32_t j) { return bpos[j]; },
                         [&](const V4<R>& o, uint32_t) {
                             R dx, dy, dz;
@@ -853,7 +915,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
                         });
             }
             const uint32_t len = lists ? NL.fcount[t] : kListOverflow;
-            walk<R>(NL.f, len, B.n_total, t, fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
+            walk<R>(NL.f, len, B.n_total, t, FT, c, [&](uint32_t j) { return xp[j]; },
                     [&](const V4<R>& o, uint32_t) {
                         R dx, dy, dz;
                         pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
@@ -879,7 +941,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
         R pd = 0;
         pd += binit[b];
         const uint32_t len = lists ? NL.bfcount[b] : kListOverflow;
-        walk<R>(NL.bf, len, B.nb_total, b, fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
+        walk<R>(NL.bf, len, B.nb_total, b, FT, c, [&](uint32_t j) { return xp[j]; },
                 [&](const V4<R>& o, uint32_t) {
                     R dx, dy, dz;
                     pd += poly6<R>(dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
@@ -1079,9 +1141,9 @@ __global__ __launch_bounds__(kTileThreads) void k_pressure_tile(
             });
         } else {
             if (P.nb)
-                walk_buckets<float>(bc, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                walk_buckets<float>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
                                     [&](const float4& o, uint32_t) { add(o); });
-            walk_buckets<float>(fc, c, P.grid, [&](uint32_t j) { return xp[j]; },
+            walk_buckets<float>(cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xp[j]; },
                                 [&](const float4& o, uint32_t) { add(o); });
         }
         pd *= P.density_scale;
@@ -1194,11 +1256,11 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
         const bool lists = NL.on != 0;
         if (P.nb) {
             const uint32_t len = lists ? NL.bcount[t] : kListOverflow;
-            walk<R>(NL.b, len, B.n_total, t, bcells + P.bucket_base, c, P.grid,
+            walk<R>(NL.b, len, B.n_total, t, cell_table(P, bcells, B.bgrid), c,
                     [&](uint32_t j) { return bxf[j]; }, [&](const V4<R>& o, uint32_t) { add(o); });
         }
         const uint32_t len = lists ? NL.fcount[t] : kListOverflow;
-        walk<R>(NL.f, len, B.n_total, t, fcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return xf[j]; },
+        walk<R>(NL.f, len, B.n_total, t, cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xf[j]; },
                 [&](const V4<R>& o, uint32_t) { add(o); });
         V4<R> f;
         f.x = fx * P.pf_scale;
@@ -1261,7 +1323,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
-    const uint2* fc = fcells + P.bucket_base;
+    const CellTable FT = cell_table(P, fcells, B.fgrid);
     const uint32_t t = rg.begin + threadIdx.x;
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
     const uint32_t i = boundary || t >= rg.end ? t : particle_of(B, t);
@@ -1273,7 +1335,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
             const int3 c = cell_of4<R>(self, P.cell);
             R sw = 0, sz = 0;
             if (P.nb)
-                walk_buckets<R>(bcells + P.bucket_base, c, P.grid, [&](uint32_t j) { return bpos[j]; },
+                walk_buckets<R>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
                                 [&](const V4<R>& o, uint32_t) {
                                     R dx, dy, dz;
                                     sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
@@ -1282,7 +1344,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
             struct Pair {
                 V4<R> w, z;
             };
-            walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
+            walk_buckets<R>(FT, c, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
                             [&](const Pair& o, uint32_t) {
                                 R dx, dy, dz;
                                 sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.w.x, o.w.y, o.w.z, dx, dy, dz), P.h2);
@@ -1321,7 +1383,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
         struct Pair {
             V4<R> w, z;
         };
-        walk_buckets<R>(fc, c, P.grid, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
+        walk_buckets<R>(FT, c, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
                         [&](const Pair& o, uint32_t) {
                             R dx, dy, dz;
                             sw += poly6<R>(dist2<R>(self.x, self.y, self.z, o.w.x, o.w.y, o.w.z, dx, dy, dz), P.h2);

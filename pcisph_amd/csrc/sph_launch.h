@@ -43,6 +43,10 @@ struct DeviceArrays {
     uint32_t* bkeys[2] = {nullptr, nullptr};
     uint32_t* bvals[2] = {nullptr, nullptr};
     uint2* bcells = nullptr;
+    // dense range grids of the scenes' cell boxes (CellTable), `box_cells` entries each
+    uint2* fgrid = nullptr;
+    uint2* bgrid = nullptr;
+    uint32_t box_cells = 0;
     NeighborLists NL;  // on == 0: no lists this step
     bool tile_pressure = false;  // FP32 fluid pressure passes from LDS tiles (256-thread workgroups)
 };
@@ -97,6 +101,13 @@ struct DataLaunch {
                        uint32_t* vals);
     // every scene's boundary positions += shift[3 s .. 3 s + 2] (device array)
     static void translate_scenes(const Launch& L, const Batch<R>& B, const float* shift, V4<R>* bpos);
+    // the range grids of all `scene_count` scenes' boxes (the largest has `max_cells`
+    // cells) from a bucket table
+    static void fill_box(const Launch& L, const SphParams<R>* scenes, uint32_t scene_count, uint32_t max_cells,
+                         const uint2* cells, uint2* grid);
+    // cell extents of every scene's fluid and boundary particles into ext[12 S]
+    // (k_extents; ext prepared by the caller)
+    static void extents(const Launch& L, const Batch<R>& B, const V4<R>* pos, const V4<R>* bpos, int* ext);
 };
 
 void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);

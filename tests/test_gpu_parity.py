@@ -6,6 +6,8 @@ hardware reciprocal/sqrt) must stay within a stated relative tolerance.  The sor
 products (keys, permutation, cell table) and neighbour counts are integers and are
 always compared bit for bit.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -88,6 +90,42 @@ def test_fast_fp32_within_tolerance(gpu, oracle_lib, scene_idx):
               f"ieee-vs-fp64 {errs[name][2]:.2e}", flush=True)
     bad = {k: v for k, v in errs.items() if not v[0] < FAST_RTOL}
     assert not bad, bad
+
+
+def test_cell_box_edges_bit_exact(gpu, oracle_lib):
+    """Neighbourhood lookups read a dense grid of bucket ranges inside each scene's
+    cell box (set from the particles' extents when positions are uploaded) and the
+    hash-indexed bucket table outside it.  Fluid far from its container (so the box
+    covers the container alone), fluid just inside and just beyond the box's two-cell
+    margin, and walls moved down out of the box after it was set must all match the
+    oracle bit for bit."""
+    base = small_scenes()[1]
+    h = 4.0 * base.particle_radius
+    fl = base.fluid.reshape(-1, 3)
+    bd = base.boundary.reshape(-1, 3)
+    lo, hi = bd.min(0), bd.max(0)
+    mid = (lo + hi) / 2
+    rng = np.random.default_rng(7)
+    far = np.float32(40.0) + rng.uniform(0, h, (6, 3))
+    near = np.array([hi[0] + 1.5 * h, mid[1], mid[2]]) + rng.uniform(0, 0.5 * h, (4, 3))
+    beyond = np.array([hi[0] + 3.2 * h, mid[1], mid[2]]) + rng.uniform(0, 0.5 * h, (4, 3))
+    fluid = np.concatenate([fl, far, near, beyond]).astype(np.float32).ravel()
+    scene = dataclasses.replace(base, fluid=fluid, name="box_edges")
+    print(f"case cell_box_edges N={scene.fluid_count} Nb={scene.boundary_count}", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 1, max_iterations=20)
+    sg = run(g, scene, 1, max_iterations=20)
+    g.translate_boundary(0.0, -5.0 * h, 0.0)  # walls leave the box; it is not re-measured
+    o.upload(Field.BOUNDARY_POSITIONS, g.download(Field.BOUNDARY_POSITIONS))
+    so += [o.step() for _ in range(2)]
+    sg += [g.step() for _ in range(2)]
+    for a, b in zip(so, sg):
+        for key in ("iterations", "max_density_variation", "fluid_candidates", "boundary_candidates"):
+            assert a[key] == b[key], (key, a[key], b[key])
+    ro, rg = snapshot(o), snapshot(g)
+    for name in ro:
+        np.testing.assert_array_equal(rg[name], ro[name], err_msg=name)
 
 
 def test_iteration_bounds(gpu, oracle_lib):
