@@ -387,6 +387,11 @@ class Engine final : public EngineBase {
     }
 
     void free_internal() {
+        dfree(canon_);
+        dfree(stor_);
+        dfree(srcc_);
+        dfree(code_);
+        dfree(ccells_);
         dfree(A_.cells);
         dfree(A_.bcells);
         dfree(A_.fgrid);
@@ -421,8 +426,13 @@ class Engine final : public EngineBase {
         const std::vector<uint32_t> f_off = offsets_of(n), b_off = offsets_of(nb);
         const uint32_t N = f_off.back(), Nb = b_off.back();
         if (f_off != f_off_) {
+            canonicalize();  // scenes keep their first particles in the reference's order
             regrow(A_.pos_in, N, f_off_, f_off);
             regrow(A_.vel_in, N, f_off_, f_off);
+            dalloc(canon_, N, stream_);
+            dalloc(stor_, N, stream_);
+            fill_iota(Launch{stream_, 256}, canon_, N);
+            fill_iota(Launch{stream_, 256}, stor_, N);
             for (auto** q : {&A_.pos, &A_.vel, &A_.xr, &A_.nrm, &A_.fo, &A_.fp, &A_.xp, &A_.xf}) dalloc(*q, N, stream_);
             dalloc(A_.p, N, stream_);
             dalloc(A_.dv, N, stream_);
@@ -532,6 +542,7 @@ class Engine final : public EngineBase {
     }
 
     void snapshot() override {
+        canonicalize();
         dalloc(snap_pos_, std::max(A_.n, 1u), stream_);
         dalloc(snap_vel_, std::max(A_.n, 1u), stream_);
         dalloc(snap_b_, std::max(A_.nb, 1u), stream_);
@@ -546,6 +557,12 @@ class Engine final : public EngineBase {
         HIP_CHECK(hipMemcpyAsync(A_.pos_in, snap_pos_, sizeof(V4<R>) * A_.n, hipMemcpyDeviceToDevice, stream_));
         HIP_CHECK(hipMemcpyAsync(A_.vel_in, snap_vel_, sizeof(V4<R>) * A_.n, hipMemcpyDeviceToDevice, stream_));
         HIP_CHECK(hipMemcpyAsync(A_.bpos, snap_b_, sizeof(V4<R>) * A_.nb, hipMemcpyDeviceToDevice, stream_));
+        if (layout_active_) {  // the snapshot is in canonical order
+            fill_iota(Launch{stream_, 256}, canon_, A_.n);
+            fill_iota(Launch{stream_, 256}, stor_, A_.n);
+            HIP_CHECK(hipGetLastError());
+            layout_active_ = false;
+        }
         HIP_CHECK(hipStreamSynchronize(stream_));
         boxes_dirty_ = true;
     }
@@ -568,38 +585,69 @@ class Engine final : public EngineBase {
         size_t raw_words = 0;
         uint32_t raw_bias = 0;          // subtracted from every word of a host download
         bool raw_pairs = false;         // (start, end) pairs: (0, 0) is an empty bucket
+        const uint32_t* map = nullptr;  // element i of the public array is device element map[i]
+        uint32_t raw_item_words = 0;    // uint32 words per element of a mapped uint32 field
     };
+
+    // The canonical bucket ranges (the reference's cell table) of the last sort, from
+    // its keys; a layout step keeps only the ranges in slots.
+    const uint2* canonical_cells() {
+        if (ccells_b_ != A_.B || !ccells_) {
+            dalloc(ccells_, std::max(A_.B, 1u), stream_);
+            ccells_b_ = A_.B;
+        }
+        const Launch L{stream_, 256};
+        fill_u32(L, (uint32_t*)ccells_, 2 * (size_t)A_.B, 0u);
+        cells_from_keys(L, A_.n, A_.keys[fluid_sorted_], ccells_);
+        HIP_CHECK(hipGetLastError());
+        return ccells_;
+    }
 
     FieldView view(uint32_t s, int field) {
         FieldView v;
         const uint32_t f = f_off_[s], b = b_off_[s], n = n_[s], nb = nb_[s];
         const uint32_t B = s < buckets_.size() ? buckets_[s] : 0, base = s < buckets_.size() ? bucket_base_[s] : 0;
+        // fluid per-particle arrays: through the slot map while the arrays are in a layout
+        const uint32_t fo = layout_active_ ? 0u : f;
+        if (layout_active_) v.map = stor_ + f;
         switch (field) {
-            case PCISPH_FLUID_POSITIONS: v.vec = A_.pos_in + f; v.n = n; break;
-            case PCISPH_FLUID_VELOCITIES: v.vec = A_.vel_in + f; v.n = n; break;
-            case PCISPH_FLUID_NORMALS: v.vec = A_.nrm + f; v.n = n; break;
-            case PCISPH_FLUID_PREDICTED_POSITIONS: v.vec = A_.xp + f; v.n = n; break;
-            case PCISPH_FLUID_OTHER_FORCES: v.vec = A_.fo + f; v.n = n; break;
-            case PCISPH_FLUID_PRESSURE_FORCES: v.vec = A_.fp + f; v.n = n; break;
-            case PCISPH_FLUID_DENSITIES: v.scalar = (R*)(A_.xr + f); v.stride = 4; v.offset = 3; v.n = n; break;
-            case PCISPH_FLUID_PRESSURES: v.scalar = A_.p + f; v.n = n; break;
-            case PCISPH_FLUID_DENSITY_VARIATIONS: v.scalar = A_.dv + f; v.n = n; break;
-            case PCISPH_BOUNDARY_POSITIONS: v.vec = A_.bpos + b; v.n = nb; break;
-            case PCISPH_BOUNDARY_PRESSURES: v.scalar = A_.bp + b; v.n = nb; break;
-            case PCISPH_BOUNDARY_INIT_PRED_DENSITIES: v.scalar = A_.binit + b; v.n = nb; break;
-            case PCISPH_FLUID_KEYS: v.raw = A_.keys[fluid_sorted_] + f; v.raw_words = n; v.raw_bias = base; break;
-            case PCISPH_FLUID_SRC_LOCATIONS: v.raw = A_.vals[fluid_sorted_] + f; v.raw_words = n; v.raw_bias = f; break;
-            case PCISPH_FLUID_CELL_OFFSETS:
-                v.raw = (const uint32_t*)(A_.cells + base); v.raw_words = 2 * (size_t)B; v.raw_bias = f;
-                v.raw_pairs = true;
+            case PCISPH_FLUID_POSITIONS: v.vec = A_.pos_in + fo; v.n = n; break;
+            case PCISPH_FLUID_VELOCITIES: v.vec = A_.vel_in + fo; v.n = n; break;
+            case PCISPH_FLUID_NORMALS: v.vec = A_.nrm + fo; v.n = n; break;
+            case PCISPH_FLUID_PREDICTED_POSITIONS: v.vec = A_.xp + fo; v.n = n; break;
+            case PCISPH_FLUID_OTHER_FORCES: v.vec = A_.fo + fo; v.n = n; break;
+            case PCISPH_FLUID_PRESSURE_FORCES: v.vec = A_.fp + fo; v.n = n; break;
+            case PCISPH_FLUID_DENSITIES: v.scalar = (R*)(A_.xr + fo); v.stride = 4; v.offset = 3; v.n = n; break;
+            case PCISPH_FLUID_PRESSURES: v.scalar = A_.p + fo; v.n = n; break;
+            case PCISPH_FLUID_DENSITY_VARIATIONS: v.scalar = A_.dv + fo; v.n = n; break;
+            case PCISPH_BOUNDARY_POSITIONS: v.vec = A_.bpos + b; v.n = nb; v.map = nullptr; break;
+            case PCISPH_BOUNDARY_PRESSURES: v.scalar = A_.bp + b; v.n = nb; v.map = nullptr; break;
+            case PCISPH_BOUNDARY_INIT_PRED_DENSITIES: v.scalar = A_.binit + b; v.n = nb; v.map = nullptr; break;
+            // sort products are in canonical order
+            case PCISPH_FLUID_KEYS:
+                v.raw = A_.keys[fluid_sorted_] + f; v.raw_words = n; v.raw_bias = base; v.map = nullptr;
                 break;
-            case PCISPH_BOUNDARY_KEYS: v.raw = A_.bkeys[boundary_sorted_] + b; v.raw_words = nb; v.raw_bias = base; break;
-            case PCISPH_BOUNDARY_SRC_LOCATIONS: v.raw = A_.bvals[boundary_sorted_] + b; v.raw_words = nb; v.raw_bias = b; break;
+            case PCISPH_FLUID_SRC_LOCATIONS:
+                v.raw = (sorted_layout_ ? srcc_ : A_.vals[fluid_sorted_]) + f; v.raw_words = n; v.raw_bias = f;
+                v.map = nullptr;
+                break;
+            case PCISPH_FLUID_CELL_OFFSETS:
+                v.raw = (const uint32_t*)((sorted_layout_ ? canonical_cells() : A_.cells) + base);
+                v.raw_words = 2 * (size_t)B; v.raw_bias = f; v.raw_pairs = true; v.map = nullptr;
+                break;
+            case PCISPH_BOUNDARY_KEYS:
+                v.raw = A_.bkeys[boundary_sorted_] + b; v.raw_words = nb; v.raw_bias = base; v.map = nullptr;
+                break;
+            case PCISPH_BOUNDARY_SRC_LOCATIONS:
+                v.raw = A_.bvals[boundary_sorted_] + b; v.raw_words = nb; v.raw_bias = b; v.map = nullptr;
+                break;
             case PCISPH_BOUNDARY_CELL_OFFSETS:
                 v.raw = (const uint32_t*)(A_.bcells + base); v.raw_words = nb ? 2 * (size_t)B : 0; v.raw_bias = b;
-                v.raw_pairs = true;
+                v.raw_pairs = true; v.map = nullptr;
                 break;
-            case PCISPH_FLUID_NEIGHBOR_COUNTS: v.raw = (const uint32_t*)(A_.counts + f); v.raw_words = 4 * (size_t)n; break;
+            case PCISPH_FLUID_NEIGHBOR_COUNTS:
+                v.raw = (const uint32_t*)(A_.counts + fo); v.raw_words = 4 * (size_t)n; v.raw_item_words = 4;
+                break;
             default: throw Error(PCISPH_ERR_INVALID_ARGUMENT, "unknown field " + std::to_string(field));
         }
         return v;
@@ -616,8 +664,8 @@ class Engine final : public EngineBase {
             HIP_CHECK(hipMemcpyAsync(staging_, src, count * sizeof(float), hipMemcpyHostToDevice, stream_));
             from = (const float*)staging_;
         }
-        if (v.vec) DataLaunch<R>::unpack_vec3(L, from, v.vec, v.n);
-        else DataLaunch<R>::unpack_scalar(L, from, v.scalar, v.n, v.stride, v.offset);
+        if (v.vec) DataLaunch<R>::unpack_vec3(L, from, v.vec, v.n, v.map);
+        else DataLaunch<R>::unpack_scalar(L, from, v.scalar, v.n, v.stride, v.offset, v.map);
         if (field == PCISPH_FLUID_POSITIONS || field == PCISPH_BOUNDARY_POSITIONS) boxes_dirty_ = true;
         HIP_CHECK(hipGetLastError());
         if (!device) HIP_CHECK(hipStreamSynchronize(stream_));  // staging reuse + host buffer lifetime
@@ -633,7 +681,14 @@ class Engine final : public EngineBase {
             if (device && v.raw_bias)
                 throw Error(PCISPH_ERR_INVALID_ARGUMENT,
                             "device download of a sort product is available for the first scene only");
-            HIP_CHECK(hipMemcpyAsync(dst, v.raw, count * 4, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+            const uint32_t* from = v.raw;
+            if (v.map) {  // a per-particle field in a layout: gathered into canonical order first
+                gather_words(L, v.raw_item_words ? (uint32_t)(count / v.raw_item_words) : 0u, v.raw_item_words,
+                             v.map, v.raw, (uint32_t*)staging_);
+                HIP_CHECK(hipGetLastError());
+                from = (const uint32_t*)staging_;
+            }
+            HIP_CHECK(hipMemcpyAsync(dst, from, count * 4, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                      stream_));
             HIP_CHECK(hipStreamSynchronize(stream_));
             if (v.raw_bias) {  // batch-wide indices and keys to the scene's own
@@ -649,11 +704,11 @@ class Engine final : public EngineBase {
         }
         void* to = device ? dst : (void*)staging_;
         if (f64) {
-            if (v.vec) DataLaunch<R>::pack_vec3_f64(L, v.vec, (double*)to, v.n);
-            else DataLaunch<R>::pack_scalar_f64(L, v.scalar, (double*)to, v.n, v.stride, v.offset);
+            if (v.vec) DataLaunch<R>::pack_vec3_f64(L, v.vec, (double*)to, v.n, v.map);
+            else DataLaunch<R>::pack_scalar_f64(L, v.scalar, (double*)to, v.n, v.stride, v.offset, v.map);
         } else {
-            if (v.vec) DataLaunch<R>::pack_vec3(L, v.vec, (float*)to, v.n);
-            else DataLaunch<R>::pack_scalar(L, v.scalar, (float*)to, v.n, v.stride, v.offset);
+            if (v.vec) DataLaunch<R>::pack_vec3(L, v.vec, (float*)to, v.n, v.map);
+            else DataLaunch<R>::pack_scalar(L, v.scalar, (float*)to, v.n, v.stride, v.offset, v.map);
         }
         HIP_CHECK(hipGetLastError());
         if (!device)
@@ -897,8 +952,8 @@ class Engine final : public EngineBase {
                               (const void*)D_.p0, (const void*)D_.dv0, (const void*)D_.bp0, (const void*)D_.bxf0})
             ptr(p);
         const Batch<R>& B = A_.batch;
-        ptr(B.scenes), ptr(B.blocks), ptr(B.states), ptr(B.order), ptr(okeys_[0]), ptr(okeys_[1]),
-            ptr(ovals_[0]), ptr(ovals_[1]);
+        ptr(B.scenes), ptr(B.blocks), ptr(B.states), ptr(okeys_[0]), ptr(okeys_[1]), ptr(ovals_[0]),
+            ptr(ovals_[1]), ptr(canon_), ptr(stor_), ptr(srcc_), ptr(code_);
         put(B.scene_count), put(B.fluid_blocks), put(B.boundary_blocks), put(B.n_total), put(B.nb_total);
         const NeighborLists& NL = A_.NL;
         ptr(NL.f), ptr(NL.b), ptr(NL.bf), ptr(NL.fcount), ptr(NL.bcount), ptr(NL.bfcount);
@@ -930,12 +985,11 @@ class Engine final : public EngineBase {
         // work order: Morton codes with `obits` bits per axis below the scene index
         const uint32_t sbits = key_bits_for(S);
         const uint32_t obits = std::min(7u, sbits < 32 ? (32u - sbits) / 3u : 0u);
-        const bool ordered = (o.work_order & 1) && obits >= 2 && A_.n > 1;
-        const uint32_t order_bits = ordered ? 3 * obits + sbits : 0;
-        if (ordered) ensure_order_buffers();
-        const uint32_t order_passes = (order_bits + digit - 1) / digit;
-        const uint32_t* order = ordered ? ovals_[order_passes & 1] : nullptr;
-        A_.batch = Batch<R>{scene_table_, block_table_, states_, order, S, fluid_blocks_, boundary_blocks_,
+        const bool layout = (o.work_order & 1) && obits >= 2 && A_.n > 1;
+        const uint32_t layout_bits = layout ? 3 * obits + sbits : 0;
+        if (!layout) canonicalize();
+        if (layout) ensure_order_buffers();
+        A_.batch = Batch<R>{scene_table_, block_table_, states_, S, fluid_blocks_, boundary_blocks_,
                             A_.n, A_.nb, 0, A_.box_cells ? A_.fgrid : nullptr,
                             A_.box_cells ? A_.bgrid : nullptr};
         A_.tile_pressure = (o.work_order & 4) != 0;
@@ -980,20 +1034,39 @@ class Engine final : public EngineBase {
         auto prologue = [&] {
             prof_.timed(PCISPH_STAGE_HASH, stream_, -1, [&] {
                 fill_u32(L, (uint32_t*)A_.cells, 2 * (size_t)A_.B, 0u);
-                DataLaunch<R>::hash(L, A_.batch, false, A_.pos_in, A_.keys[0], A_.vals[0]);
+                DataLaunch<R>::hash(L, A_.batch, false, A_.pos_in, A_.keys[0], A_.vals[0], layout ? canon_ : nullptr);
             });
             prof_.timed(PCISPH_STAGE_SORT, stream_, -1, [&] {
                 fluid_sorted_ = radix_sort(L, A_.keys, A_.vals, A_.n, key_bits, digit, sort_scratch_);
             });
             if (fluid_sorted_ < 0) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "too many particles for one sort");
             prof_.timed(PCISPH_STAGE_REORDER, stream_, -1, [&] {
-                DataLaunch<R>::reorder(L, A_.n, A_.keys[fluid_sorted_], A_.vals[fluid_sorted_], A_.pos_in,
-                                       A_.vel_in, A_.pos, A_.vel, A_.cells);
-                if (ordered) {
-                    DataLaunch<R>::morton(L, A_.batch, A_.pos, obits, okeys_[0], ovals_[0]);
-                    const int res = radix_sort(L, okeys_, ovals_, A_.n, order_bits, digit, sort_scratch_);
-                    if (res != (int)(order_passes & 1))
-                        throw Error(PCISPH_ERR_INTERNAL, "work-order sort ended in an unexpected buffer");
+                if (layout) {
+                    LayoutArgs<R> a{};
+                    a.K = A_.keys[fluid_sorted_];
+                    a.S = A_.vals[fluid_sorted_];
+                    a.canon = canon_;
+                    a.stor = stor_;
+                    a.srcc = srcc_;
+                    a.code = code_;
+                    for (int k = 0; k < 2; ++k) {
+                        a.okeys[k] = okeys_[k];
+                        a.ovals[k] = ovals_[k];
+                    }
+                    a.bits = obits;
+                    a.key_bits = layout_bits;
+                    a.digit_bits = digit;
+                    a.scratch = sort_scratch_;
+                    a.pin = A_.pos_in;
+                    a.vin = A_.vel_in;
+                    a.pout = A_.pos;
+                    a.vout = A_.vel;
+                    a.cells = A_.cells;
+                    if (DataLaunch<R>::layout(L, A_.batch, a) < 0)
+                        throw Error(PCISPH_ERR_INTERNAL, "layout sort failed");
+                } else {
+                    DataLaunch<R>::reorder(L, A_.n, A_.keys[fluid_sorted_], A_.vals[fluid_sorted_], A_.pos_in,
+                                           A_.vel_in, A_.pos, A_.vel, A_.cells);
                 }
                 if (A_.box_cells) DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.cells, A_.fgrid);
             });
@@ -1031,6 +1104,8 @@ class Engine final : public EngineBase {
             prologue();
         }
         HIP_CHECK(hipGetLastError());
+        layout_active_ = layout;
+        sorted_layout_ = layout;
 
         // Iterations are enqueued `step` at a time (one graph, or one iteration), keeping
         // at least one iteration queued beyond the one the host waits for.
@@ -1134,13 +1209,43 @@ class Engine final : public EngineBase {
         NL.on = 1;
     }
 
+    // Buffers of the Morton layout: its sort, the bucket codes, the reference's
+    // source locations.  The slot maps canon_ / stor_ live with the fluid arrays.
     void ensure_order_buffers() {
-        if (order_n_ == A_.n && okeys_[0]) return;
-        for (int k = 0; k < 2; ++k) {
-            dalloc(okeys_[k], std::max(A_.n, 1u), stream_);
-            dalloc(ovals_[k], std::max(A_.n, 1u), stream_);
+        if (order_n_ != A_.n || !okeys_[0]) {
+            for (int k = 0; k < 2; ++k) {
+                dalloc(okeys_[k], std::max(A_.n, 1u), stream_);
+                dalloc(ovals_[k], std::max(A_.n, 1u), stream_);
+            }
+            dalloc(srcc_, std::max(A_.n, 1u), stream_);
+            order_n_ = A_.n;
         }
-        order_n_ = A_.n;
+        if (code_b_ != A_.B || !code_) {
+            dalloc(code_, std::max(A_.B, 1u), stream_);
+            code_b_ = A_.B;
+        }
+    }
+
+    // The public fluid arrays (and this step's per-particle results) back in
+    // canonical order, identity slot maps.
+    void canonicalize() {
+        if (!layout_active_) return;
+        const Launch L{stream_, 256};
+        const uint32_t n = A_.n;
+        auto permute = [&](void* p, uint32_t words) {
+            if (!p) return;
+            gather_words(L, n, words, stor_, (const uint32_t*)p, (uint32_t*)A_.pos);  // A_.pos: scratch here
+            HIP_CHECK(hipMemcpyAsync(p, A_.pos, (size_t)n * words * 4, hipMemcpyDeviceToDevice, stream_));
+        };
+        constexpr uint32_t v4 = sizeof(V4<R>) / 4, r1 = sizeof(R) / 4;
+        for (V4<R>* q : {A_.pos_in, A_.vel_in, A_.xr, A_.nrm, A_.fo, A_.fp, A_.xp, A_.xf}) permute(q, v4);
+        for (R* q : {A_.p, A_.dv}) permute(q, r1);
+        permute(A_.counts, 4);
+        fill_iota(L, canon_, n);
+        fill_iota(L, stor_, n);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        layout_active_ = false;
     }
 
     void ensure_warm_buffers(bool dual) {
@@ -1205,8 +1310,16 @@ class Engine final : public EngineBase {
     float* feat_ = nullptr;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
     float list_skin_ = 0.f;
-    uint32_t* okeys_[2] = {nullptr, nullptr};  // work-order sort: keys
-    uint32_t* ovals_[2] = {nullptr, nullptr};  // and slots (the result is the order)
+    uint32_t* okeys_[2] = {nullptr, nullptr};  // layout sort: bucket codes
+    uint32_t* ovals_[2] = {nullptr, nullptr};  // and canonical indices (the result is the layout)
+    uint32_t* canon_ = nullptr;                // slot -> canonical index of the public fluid arrays
+    uint32_t* stor_ = nullptr;                 // canonical index -> slot
+    uint32_t* srcc_ = nullptr;                 // the reference's source locations of the last layout step
+    uint32_t* code_ = nullptr;                 // layout code of every bucket of the batch
+    uint2* ccells_ = nullptr;                  // canonical bucket ranges, built for downloads
+    uint32_t code_b_ = 0, ccells_b_ = 0;
+    bool layout_active_ = false;               // the public fluid arrays are in a Morton layout
+    bool sorted_layout_ = false;               // the last sort's products come from a layout step
     uint32_t order_n_ = 0;
     uint32_t list_buckets_ = 0;
     uint32_t* ctl_ = nullptr;  // device word: the tag of the step in flight

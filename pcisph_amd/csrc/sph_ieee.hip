@@ -9,8 +9,11 @@ namespace pcisph {
 
 namespace {
 
+// Layout conversion between the public float[3n] / float[n] arrays (element i) and
+// the device arrays (element map[i], or i without a map).
 template <typename R>
-__global__ void k_unpack_vec3(const float* __restrict__ src, V4<R>* __restrict__ dst, uint32_t n) {
+__global__ void k_unpack_vec3(const float* __restrict__ src, V4<R>* __restrict__ dst, uint32_t n,
+                              const uint32_t* __restrict__ map) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     V4<R> v;
@@ -18,14 +21,15 @@ __global__ void k_unpack_vec3(const float* __restrict__ src, V4<R>* __restrict__
     v.y = (R)src[3 * (size_t)i + 1];
     v.z = (R)src[3 * (size_t)i + 2];
     v.w = 0;
-    dst[i] = v;
+    dst[map ? map[i] : i] = v;
 }
 
 template <typename R, typename O>
-__global__ void k_pack_vec3(const V4<R>* __restrict__ src, O* __restrict__ dst, uint32_t n) {
+__global__ void k_pack_vec3(const V4<R>* __restrict__ src, O* __restrict__ dst, uint32_t n,
+                            const uint32_t* __restrict__ map) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const V4<R> v = src[i];
+    const V4<R> v = src[map ? map[i] : i];
     dst[3 * (size_t)i] = (O)v.x;
     dst[3 * (size_t)i + 1] = (O)v.y;
     dst[3 * (size_t)i + 2] = (O)v.z;
@@ -33,18 +37,47 @@ __global__ void k_pack_vec3(const V4<R>* __restrict__ src, O* __restrict__ dst, 
 
 template <typename R>
 __global__ void k_unpack_scalar(const float* __restrict__ src, R* __restrict__ dst, uint32_t n,
-                                uint32_t stride, uint32_t offset) {
+                                uint32_t stride, uint32_t offset, const uint32_t* __restrict__ map) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    dst[(size_t)i * stride + offset] = (R)src[i];
+    dst[(size_t)(map ? map[i] : i) * stride + offset] = (R)src[i];
 }
 
 template <typename R, typename O>
 __global__ void k_pack_scalar(const R* __restrict__ src, O* __restrict__ dst, uint32_t n, uint32_t stride,
-                              uint32_t offset) {
+                              uint32_t offset, const uint32_t* __restrict__ map) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    dst[i] = (O)src[(size_t)i * stride + offset];
+    dst[i] = (O)src[(size_t)(map ? map[i] : i) * stride + offset];
+}
+
+// dst[i] = item map[i] of src (`words` uint32 per item)
+__global__ void k_gather_words(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, uint32_t n,
+                               uint32_t words, const uint32_t* __restrict__ map) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t w = 0; w < words; ++w) dst[(size_t)i * words + w] = src[(size_t)map[i] * words + w];
+}
+
+template <typename R>
+__global__ void k_gather_v4(const V4<R>* __restrict__ src, V4<R>* __restrict__ dst, uint32_t n,
+                            const uint32_t* __restrict__ map) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[map[i]];
+}
+
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// Bucket ranges in canonical indices from the sorted keys (the reference's cell table).
+__global__ void k_cells_from_keys(uint32_t n, const uint32_t* __restrict__ keys, uint2* __restrict__ cells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t k = keys[c];
+    if (c == 0 || k != keys[c - 1]) cells[k].x = c;
+    if (c == n - 1 || k != keys[c + 1]) cells[k].y = c + 1;
 }
 
 template <typename R>
@@ -65,20 +98,66 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v, uint32_t bits) {
     return out;
 }
 
-// Work-order keys of the sorted fluid particles: scene index above the Morton code
-// of the particle's cell (coordinates taken modulo 2^bits), value = the slot.
+// The Morton layout of the fluid particles.  The reference's arrays are in bucket
+// order (sort_utils.cl:25-78); this build stores each bucket's particles contiguously
+// and in the reference's order, but orders the buckets along a Morton curve, so the
+// candidates a neighbourhood gathers lie together in memory.  Kernels index storage
+// slots; `canon` (slot -> canonical index, the index in the reference's arrays) and
+// `stor` (its inverse) translate at the public interface.
+//
+// Part 1, over canonical indices c of the sorted keys K (S[c] = the particle's slot
+// before this step): the reference's source location of c (the canonical index the
+// particle had before the sort) and, at the first c of each bucket, the bucket's
+// layout code: scene index above the Morton code of that particle's cell (cell
+// coordinates taken modulo 2^bits).
 template <typename R>
-__global__ void k_morton(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
-                         const V4<R>* __restrict__ pos, uint32_t bits, uint32_t* __restrict__ keys,
-                         uint32_t* __restrict__ vals) {
+__global__ void k_layout_codes(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
+                               const uint32_t* __restrict__ K, const uint32_t* __restrict__ S,
+                               const uint32_t* __restrict__ canon, const V4<R>* __restrict__ pos, uint32_t bits,
+                               uint32_t* __restrict__ code, uint32_t* __restrict__ srcc) {
     const BlockRange rg = blocks[blockIdx.x];
-    const uint32_t t = rg.begin + threadIdx.x;
-    if (t >= rg.end) return;
-    const int3 c = cell_of4<R>(pos[t], scenes[rg.scene].cell);
-    const uint32_t code = spread3((uint32_t)c.x, bits) | (spread3((uint32_t)c.y, bits) << 1) |
-                          (spread3((uint32_t)c.z, bits) << 2);
-    keys[t] = (rg.scene << (3 * bits)) | code;
-    vals[t] = t;
+    const uint32_t c = rg.begin + threadIdx.x;
+    if (c >= rg.end) return;
+    const uint32_t s = S[c];
+    srcc[c] = canon[s];
+    const uint32_t k = K[c];
+    if (c == 0 || K[c - 1] != k) {
+        const int3 q = cell_of4<R>(pos[s], scenes[rg.scene].cell);
+        code[k] = (rg.scene << (3 * bits)) | spread3((uint32_t)q.x, bits) | (spread3((uint32_t)q.y, bits) << 1) |
+                  (spread3((uint32_t)q.z, bits) << 2);
+    }
+}
+
+// Part 2: every canonical index takes its bucket's code; sorted stably, the values
+// give the new layout (slot -> canonical index) with buckets whole and in the
+// reference's order inside.
+__global__ void k_layout_keys(uint32_t n, const uint32_t* __restrict__ K, const uint32_t* __restrict__ code,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    keys[c] = code[K[c]];
+    vals[c] = c;
+}
+
+// Part 3: slot t takes canonical index order[t] -- position and velocity from the
+// particle's previous slot S[order[t]] -- records both maps, and the bucket ranges
+// in slots.
+template <typename R>
+__global__ void k_layout_reorder(uint32_t n, const uint32_t* __restrict__ order, const uint32_t* __restrict__ K,
+                                 const uint32_t* __restrict__ S, const V4<R>* __restrict__ pin,
+                                 const V4<R>* __restrict__ vin, V4<R>* __restrict__ pout, V4<R>* __restrict__ vout,
+                                 uint32_t* __restrict__ canon, uint32_t* __restrict__ stor, uint2* __restrict__ cells) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t c = order[t];
+    const uint32_t s = S[c];
+    pout[t] = pin[s];
+    vout[t] = vin[s];
+    canon[t] = c;
+    stor[c] = t;
+    const uint32_t k = K[c];
+    if (t == 0 || K[order[t - 1]] != k) cells[k].x = t;
+    if (t == n - 1 || K[order[t + 1]] != k) cells[k].y = t + 1;
 }
 
 // Every boundary range moved by its scene's shift[3 * scene ..].
@@ -99,11 +178,42 @@ __global__ void k_translate_scenes(const BlockRange* __restrict__ blocks, const 
 }  // namespace
 
 template <typename R>
-void DataLaunch<R>::morton(const Launch& L, const Batch<R>& B, const V4<R>* pos, uint32_t bits, uint32_t* keys,
-                           uint32_t* vals) {
-    if (!B.fluid_blocks) return;
-    hipLaunchKernelGGL((k_morton<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B.scenes, B.blocks, pos, bits,
-                       keys, vals);
+int DataLaunch<R>::layout(const Launch& L, const Batch<R>& B, const LayoutArgs<R>& a) {
+    const uint32_t n = B.n_total;
+    if (!n || !B.fluid_blocks) return 0;
+    hipLaunchKernelGGL((k_layout_codes<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B.scenes, B.blocks,
+                       a.K, a.S, a.canon, a.pin, a.bits, a.code, a.srcc);
+    hipLaunchKernelGGL(k_layout_keys, dim3(grid_for(n, L.block)), dim3(L.block), 0, L.stream, n, a.K, a.code,
+                       a.okeys[0], a.ovals[0]);
+    uint32_t* keys[2] = {a.okeys[0], a.okeys[1]};
+    uint32_t* vals[2] = {a.ovals[0], a.ovals[1]};
+    const int res = radix_sort(L, keys, vals, n, a.key_bits, a.digit_bits, a.scratch);
+    if (res < 0) return res;
+    hipLaunchKernelGGL((k_layout_reorder<R>), dim3(grid_for(n, L.block)), dim3(L.block), 0, L.stream, n,
+                       vals[res], a.K, a.S, a.pin, a.vin, a.pout, a.vout, a.canon, a.stor, a.cells);
+    return res;
+}
+
+template <typename R>
+void DataLaunch<R>::gather_v4(const Launch& L, uint32_t n, const uint32_t* map, const V4<R>* src, V4<R>* dst) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_gather_v4<R>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n, map);
+}
+
+void gather_words(const Launch& L, uint32_t n, uint32_t words, const uint32_t* map, const uint32_t* src,
+                  uint32_t* dst) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_gather_words, dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n, words, map);
+}
+
+void fill_iota(const Launch& L, uint32_t* p, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256)), dim3(256), 0, L.stream, p, n);
+}
+
+void cells_from_keys(const Launch& L, uint32_t n, const uint32_t* keys, uint2* cells) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_cells_from_keys, dim3(grid_for(n, 256)), dim3(256), 0, L.stream, n, keys, cells);
 }
 
 template <typename R>
@@ -136,11 +246,11 @@ void DataLaunch<R>::translate(const Launch& L, V4<R>* p, uint32_t n, float dx, f
 
 template <typename R>
 void DataLaunch<R>::hash(const Launch& L, const Batch<R>& B, bool boundary, const V4<R>* pos, uint32_t* keys,
-                         uint32_t* vals) {
+                         uint32_t* vals, const uint32_t* canon) {
     const uint32_t grid = boundary ? B.boundary_blocks : B.fluid_blocks;
     if (!grid) return;
     hipLaunchKernelGGL((k_hash<R>), dim3(grid), dim3(L.block), 0, L.stream, B.scenes,
-                       B.blocks + (boundary ? B.fluid_blocks : 0u), pos, keys, vals);
+                       B.blocks + (boundary ? B.fluid_blocks : 0u), pos, keys, vals, canon);
 }
 
 template <typename R>
@@ -159,45 +269,46 @@ void DataLaunch<R>::reorder(const Launch& L, uint32_t n, const uint32_t* keys, c
 }
 
 template <typename R>
-void DataLaunch<R>::unpack_vec3(const Launch& L, const float* src, V4<R>* dst, uint32_t n) {
+void DataLaunch<R>::unpack_vec3(const Launch& L, const float* src, V4<R>* dst, uint32_t n, const uint32_t* map) {
     if (!n) return;
-    hipLaunchKernelGGL((k_unpack_vec3<R>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n);
+    hipLaunchKernelGGL((k_unpack_vec3<R>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n, map);
 }
 
 template <typename R>
-void DataLaunch<R>::pack_vec3(const Launch& L, const V4<R>* src, float* dst, uint32_t n) {
+void DataLaunch<R>::pack_vec3(const Launch& L, const V4<R>* src, float* dst, uint32_t n, const uint32_t* map) {
     if (!n) return;
-    hipLaunchKernelGGL((k_pack_vec3<R, float>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n);
+    hipLaunchKernelGGL((k_pack_vec3<R, float>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n, map);
 }
 
 template <typename R>
-void DataLaunch<R>::pack_vec3_f64(const Launch& L, const V4<R>* src, double* dst, uint32_t n) {
+void DataLaunch<R>::pack_vec3_f64(const Launch& L, const V4<R>* src, double* dst, uint32_t n,
+                                  const uint32_t* map) {
     if (!n) return;
-    hipLaunchKernelGGL((k_pack_vec3<R, double>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n);
+    hipLaunchKernelGGL((k_pack_vec3<R, double>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n, map);
 }
 
 template <typename R>
 void DataLaunch<R>::unpack_scalar(const Launch& L, const float* src, R* dst, uint32_t n, uint32_t stride,
-                                  uint32_t offset) {
+                                  uint32_t offset, const uint32_t* map) {
     if (!n) return;
     hipLaunchKernelGGL((k_unpack_scalar<R>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n,
-                       stride, offset);
+                       stride, offset, map);
 }
 
 template <typename R>
 void DataLaunch<R>::pack_scalar(const Launch& L, const R* src, float* dst, uint32_t n, uint32_t stride,
-                                uint32_t offset) {
+                                uint32_t offset, const uint32_t* map) {
     if (!n) return;
     hipLaunchKernelGGL((k_pack_scalar<R, float>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n,
-                       stride, offset);
+                       stride, offset, map);
 }
 
 template <typename R>
 void DataLaunch<R>::pack_scalar_f64(const Launch& L, const R* src, double* dst, uint32_t n, uint32_t stride,
-                                    uint32_t offset) {
+                                    uint32_t offset, const uint32_t* map) {
     if (!n) return;
     hipLaunchKernelGGL((k_pack_scalar<R, double>), dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst,
-                       n, stride, offset);
+                       n, stride, offset, map);
 }
 
 template struct DataLaunch<float>;

@@ -11,14 +11,24 @@
 //
 // Layout in HBM (one array per field, structure of arrays, 16-byte elements so a
 // candidate costs one dwordx4 load):
-//   pos   float4 (x, y, z, 0)          sorted positions of this step
+//   pos   float4 (x, y, z, 0)          positions of this step, in its storage order
 //   xr    float4 (x, y, z, rho)        written by the density pass
 //   vel, nrm, fo, fp, xp  float4       velocity, normal, other force, pressure force,
 //                                      predicted position
 //   xf    float4 (x, y, z, p/rho^2)    written by the pressure pass for the force pass
 //   bpos  float4 (x, y, z, 0), bxf float4 (x, y, z, p_b/rho0^2) for the boundary
-// Every kernel is one work-item per particle in the sorted order, so the 64 lanes of
-// a wavefront walk neighbouring buckets together and share their loads.
+// Every kernel is one work-item per particle in storage order (below), so the 64
+// lanes of a wavefront walk neighbouring buckets together and share their loads.
+//
+// Storage order.  The reference keeps its arrays in bucket order (sorted by cell
+// hash), where consecutive buckets are cells scattered over the whole domain.  Here
+// each bucket's fluid particles stay contiguous and in the reference's order, but
+// the buckets follow a Morton curve of their cells (the Morton layout, built every
+// step after the reference's sort), so the candidates a neighbourhood gathers lie
+// together in memory and a wavefront covers a compact block of cells.  A walk over
+// bucket k visits exactly the reference's candidates in the reference's order.  The
+// solver maps storage slots to canonical indices (the reference's) at the public
+// interface; with the layout off, storage order is the reference's.
 //
 // Batches.  A launch covers every scene of a batch.  The scenes' particles are
 // concatenated (scene s owns fluid [f_off, f_off + n) and boundary
@@ -138,23 +148,13 @@ struct FeatureArgs {
     const uint4* counts;  // per-particle neighbour counts of the density pass
 };
 
-// The batch as every step kernel sees it.
-//
-// Work order.  Fluid data stays in the reference's sorted (cell-hash) order, which
-// the bucket walks need and the outputs keep; but consecutive hash keys are cells
-// scattered over the whole domain, so a wavefront of consecutive particles would
-// gather from ~8 unrelated neighbourhoods.  Fluid work items therefore go through
-// `order`: slot t of a fluid range works on particle order[t], a Morton order of the
-// cells within each scene, so a wavefront covers a compact block of cells and its
-// gathers share cache lines.  Per-particle sums still visit the candidates in the
-// reference's bucket order.  Neighbour lists and their counts are kept per slot t,
-// so a wavefront reads its lists coalesced.
+// The batch as every step kernel sees it.  Neighbour lists and their counts are
+// kept per storage slot (slot-major), so a wavefront reads its lists coalesced.
 template <typename R>
 struct Batch {
     const SphParams<R>* scenes;   // [S]
     const BlockRange* blocks;     // fluid ranges [0, fluid_blocks), then boundary ranges
     LoopState* states;            // [S]
-    const uint32_t* order;        // fluid slot -> particle, or null (slot t is particle t)
     uint32_t scene_count;
     uint32_t fluid_blocks, boundary_blocks;
     uint32_t n_total, nb_total;   // list strides
@@ -176,11 +176,6 @@ __device__ __forceinline__ CellTable cell_table(const SphParams<R>& P, const uin
         T.dim[a] = grid ? P.box_dim[a] : 0u;
     }
     return T;
-}
-
-template <typename R>
-__device__ __forceinline__ uint32_t particle_of(const Batch<R>& B, uint32_t t) {
-    return B.order ? B.order[t] : t;
 }
 
 template <typename R>
@@ -419,16 +414,21 @@ constexpr int kMaxBlockThreads = 512;
 // initialize (PCISPH/data/kernels/sort_utils.cl:12-23): key = bucket_base +
 // hash(cell(pos)), src = index.  `blocks` is the fluid or the boundary part of the
 // block table.
+// With `canon` (fluid particles stored in the Morton layout) the particle in slot i
+// has canonical index canon[i] -- its index in the reference's arrays -- and its key
+// goes there, src = the slot it is read from.
 template <typename R>
 __global__ void k_hash(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
-                       const V4<R>* __restrict__ pos, uint32_t* __restrict__ keys, uint32_t* __restrict__ src) {
+                       const V4<R>* __restrict__ pos, uint32_t* __restrict__ keys, uint32_t* __restrict__ src,
+                       const uint32_t* __restrict__ canon) {
     const BlockRange rg = blocks[blockIdx.x];
     const uint32_t i = rg.begin + threadIdx.x;
     if (i >= rg.end) return;
     const SphParams<R>& P = scenes[rg.scene];
     const int3 c = cell_of4<R>(pos[i], P.cell);
-    keys[i] = P.bucket_base + hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, P.grid);
-    src[i] = i;
+    const uint32_t at = canon ? canon[i] : i;
+    keys[at] = P.bucket_base + hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, P.grid);
+    src[at] = i;
 }
 
 // reorder_and_insert_{boundary,fluid}_offsets (sort_utils.cl:25-78): gather by the
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
     unsigned long long cf_sum = 0, cb_sum = 0;
     if (!(rg.flags & kRangeBoundary)) {
         if (in) {
-            const uint32_t i = particle_of(B, t);
+            const uint32_t i = t;
             const V4<R> self = pos[i];
             const int3 c = cell_of4<R>(self, P.cell);
             R rho = 0;
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_normal(Batch<R> B, const u
     const BlockRange rg = B.blocks[blockIdx.x];
     const uint32_t t = rg.begin + threadIdx.x;
     if (t >= rg.end) return;
-    const uint32_t i = particle_of(B, t);
+    const uint32_t i = t;
     const SphParams<R> P = B.scenes[rg.scene];
     const V4<R> self = xr[i];
     const int3 c = cell_of4<R>(self, P.cell);
@@ -773,7 +773,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, con
     float moved = 0.f;
     uint32_t bucket = 0xFFFFFFFFu;
     if (t < rg.end) {
-        const uint32_t i = particle_of(B, t);
+        const uint32_t i = t;
         const V4<R> self = xr[i];
         const V4<R> sv = vel[i];
         const V4<R> sn = nrm[i];
@@ -880,7 +880,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
     const CellTable FT = cell_table(P, fcells, B.fgrid);
     const uint32_t t = rg.begin + threadIdx.x;
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
-    const uint32_t i = boundary || t >= rg.end ? t : particle_of(B, t);
+    const uint32_t i = t;
     // lists for this wavefront if every lane's pairs are all in its list; a scene
     // fewer than 1% of whose wavefronts could use them in the step's passes so far
     // stops testing from the third pass on (displacements hardly change within a step)
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
 // ------------------------------------------------------- LDS-staged pressure pass
 //
 // The fluid ranges of update_pressure (pcisph.cl:212-263) with the candidates staged
-// in LDS.  A workgroup of 256 fluid slots in the Morton work order covers a compact
+// in LDS.  A workgroup of 256 fluid slots in the Morton layout covers a compact
 // block of ~32 cells; the union of its lanes' 27-bucket neighbourhoods is ~150
 // buckets.  The workgroup gathers those bucket ids into an LDS hash set, loads each
 // bucket's predicted positions (and boundary positions) into LDS once, and every lane
@@ -1070,7 +1070,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pressure_tile(
     const uint2* bc = bcells + P.bucket_base;
     const uint32_t t = rg.begin + threadIdx.x;
     const bool act = t < rg.end;
-    const uint32_t i = act ? particle_of(B, t) : 0u;
+    const uint32_t i = act ? t : 0u;
     float4 self = make_float4(0.f, 0.f, 0.f, 0.f), pr = self;
     int3 c = make_int3(0, 0, 0);
     if (act) {
@@ -1234,7 +1234,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
     float moved = 0.f;
     uint32_t bucket = 0xFFFFFFFFu;
     if (t < rg.end) {
-        const uint32_t i = particle_of(B, t);
+        const uint32_t i = t;
         const V4<R> self = xf[i];
         const R sf = self.w;
         const int3 c = cell_of4<R>(self, P.cell);
@@ -1326,7 +1326,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
     const CellTable FT = cell_table(P, fcells, B.fgrid);
     const uint32_t t = rg.begin + threadIdx.x;
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
-    const uint32_t i = boundary || t >= rg.end ? t : particle_of(B, t);
+    const uint32_t i = t;
     unsigned long long m[4] = {0, 0, 0, 0};  // dv warm, exp warm, dv plain, exp plain
     if (!boundary) {
         if (t < rg.end) {

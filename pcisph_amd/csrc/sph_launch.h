@@ -72,33 +72,58 @@ struct SphLaunch {
     static void boundary_init(const Launch& L, const DeviceArrays<R>& A);
 };
 
+// The Morton layout of a step (k_layout_codes, k_layout_keys, k_layout_reorder):
+// from the sorted keys K and source slots S of the reference's sort, the new slot
+// order of the fluid particles, their positions and velocities moved into it, both
+// index maps and the bucket ranges in slots.
+template <typename R>
+struct LayoutArgs {
+    const uint32_t* K;
+    const uint32_t* S;
+    uint32_t* canon;  // slot -> canonical index: read (previous layout), then rewritten
+    uint32_t* stor;   // canonical index -> slot: rewritten
+    uint32_t* srcc;   // out: the reference's source locations (canonical indices before the sort)
+    uint32_t* code;   // bucket codes, one word per bucket of the batch
+    uint32_t* okeys[2];
+    uint32_t* ovals[2];
+    uint32_t bits, key_bits, digit_bits;
+    uint32_t* scratch;
+    const V4<R>* pin;
+    const V4<R>* vin;
+    V4<R>* pout;
+    V4<R>* vout;
+    uint2* cells;  // out: bucket ranges in slots (cleared by the caller)
+};
+
 // Precision-only (no arithmetic mode) helpers, instantiated in sph_ieee.hip.
 template <typename R>
 struct DataLaunch {
-    // keys = bucket_base + hash of every fluid (boundary = false) or boundary particle
+    // keys = bucket_base + hash of every fluid (boundary = false) or boundary particle,
+    // stored at the particle's canonical index when `canon` maps slots to them
     static void hash(const Launch& L, const Batch<R>& B, bool boundary, const V4<R>* pos, uint32_t* keys,
-                     uint32_t* vals);
+                     uint32_t* vals, const uint32_t* canon = nullptr);
+    // the Morton layout (LayoutArgs); returns the buffer index of the layout sort or -1
+    static int layout(const Launch& L, const Batch<R>& B, const LayoutArgs<R>& a);
+    // dst[i] = src[map[i]]
+    static void gather_v4(const Launch& L, uint32_t n, const uint32_t* map, const V4<R>* src, V4<R>* dst);
     // p = max(0, previous pressure + network output * p_ref of the scene), xf = (x, p / rho^2)
     static void warm_apply(const Launch& L, const DeviceArrays<R>& A);
     static void reorder(const Launch& L, uint32_t n, const uint32_t* keys, const uint32_t* src,
                         const V4<R>* pin, const V4<R>* vin, V4<R>* pout, V4<R>* vout, uint2* cells);
-    // float[3n] <-> V4<R> (w set to 0 on the way in)
-    static void unpack_vec3(const Launch& L, const float* src, V4<R>* dst, uint32_t n);
-    static void pack_vec3(const Launch& L, const V4<R>* src, float* dst, uint32_t n);
-    static void pack_vec3_f64(const Launch& L, const V4<R>* src, double* dst, uint32_t n);
+    // float[3n] <-> V4<R> (w set to 0 on the way in); element i of the public array
+    // is device element map[i] (i without a map)
+    static void unpack_vec3(const Launch& L, const float* src, V4<R>* dst, uint32_t n, const uint32_t* map);
+    static void pack_vec3(const Launch& L, const V4<R>* src, float* dst, uint32_t n, const uint32_t* map);
+    static void pack_vec3_f64(const Launch& L, const V4<R>* src, double* dst, uint32_t n, const uint32_t* map);
     // float[n] <-> every `stride`-th R starting at `offset`
     static void unpack_scalar(const Launch& L, const float* src, R* dst, uint32_t n, uint32_t stride,
-                              uint32_t offset);
+                              uint32_t offset, const uint32_t* map);
     static void pack_scalar(const Launch& L, const R* src, float* dst, uint32_t n, uint32_t stride,
-                            uint32_t offset);
+                            uint32_t offset, const uint32_t* map);
     static void pack_scalar_f64(const Launch& L, const R* src, double* dst, uint32_t n, uint32_t stride,
-                                uint32_t offset);
+                                uint32_t offset, const uint32_t* map);
     // p[i].xyz += (dx, dy, dz) in precision R
     static void translate(const Launch& L, V4<R>* p, uint32_t n, float dx, float dy, float dz);
-    // work-order keys (scene, Morton code of the cell with `bits` bits per axis) and
-    // slot values of the sorted fluid particles
-    static void morton(const Launch& L, const Batch<R>& B, const V4<R>* pos, uint32_t bits, uint32_t* keys,
-                       uint32_t* vals);
     // every scene's boundary positions += shift[3 s .. 3 s + 2] (device array)
     static void translate_scenes(const Launch& L, const Batch<R>& B, const float* shift, V4<R>* bpos);
     // the range grids of all `scene_count` scenes' boxes (the largest has `max_cells`
@@ -111,6 +136,12 @@ struct DataLaunch {
 };
 
 void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);
+void fill_iota(const Launch& L, uint32_t* p, uint32_t n);  // p[i] = i
+// dst[i] = item map[i] of src, `words` uint32 per item
+void gather_words(const Launch& L, uint32_t n, uint32_t words, const uint32_t* map, const uint32_t* src,
+                  uint32_t* dst);
+// bucket ranges (c, c + 1 transitions) of sorted keys into cells (cleared by the caller)
+void cells_from_keys(const Launch& L, uint32_t n, const uint32_t* keys, uint2* cells);
 void fill_u64(const Launch& L, unsigned long long* p, size_t n, unsigned long long value);
 
 // Stable LSD radix sort of (key, value) pairs on keys < 2^key_bits.  Sorts from
