@@ -107,10 +107,13 @@ typedef struct pcisph_solver_options {
                                        whenever a launch argument changes; results are
                                        identical to 0 (launch by launch).  Default 4.  Stage
                                        profiling launches eagerly whatever this says. */
-    uint32_t work_order;            /* bit 0: fluid work items follow a per-step Morton order
-                                       of the cells (data and results stay in the reference's
-                                       sorted order; sums are unchanged), so a wavefront
-                                       covers a compact block of cells; bit 1: workgroups are
+    uint32_t work_order;            /* bit 0: fluid particles are stored in a Morton layout
+                                       rebuilt every step -- each bucket whole and in the
+                                       reference's order, the buckets along a Morton curve of
+                                       their cells -- so a wavefront covers a compact block of
+                                       cells and its gathers lie together in memory; sums,
+                                       downloads and sort products are unchanged (fields are
+                                       presented in the reference's order); bit 1: workgroups are
                                        placed so that each of the 8 XCDs (own L2) works on
                                        compact chunks of that order; bit 2: FP32 pressure
                                        passes with 256-thread workgroups stage each

@@ -933,11 +933,13 @@ class Engine final : public EngineBase {
 
     // Every value a recorded launch bakes in; graphs are re-captured when it changes.
     std::vector<uint64_t> launch_key(const Launch& L, bool fast, bool warm, bool features, uint32_t digit,
-                                     uint32_t key_bits, uint32_t maxit, uint32_t chunk, const float* mlp) const {
+                                     uint32_t key_bits, uint32_t maxit, uint32_t chunk, uint32_t layout_bits,
+                                     uint32_t morton_bits, const float* mlp) const {
         std::vector<uint64_t> k;
         auto put = [&](uint64_t v) { k.push_back(v); };
         auto ptr = [&](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
         put(L.block), put(fast), put(warm), put(features), put(digit), put(key_bits), put(maxit), put(chunk);
+        put(layout_bits), put(morton_bits);
         put(A_.tile_pressure);
         put(A_.n), put(A_.nb), put(A_.B);
         for (const void* p : {(const void*)A_.pos_in, (const void*)A_.vel_in, (const void*)A_.pos,
@@ -1000,7 +1002,8 @@ class Engine final : public EngineBase {
         // graphs record launches; the stage profiler times them one by one
         const uint32_t chunk = prof_.on() ? 0u : o.graph_iterations;
         if (chunk) {
-            std::vector<uint64_t> key = launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, w.params);
+            std::vector<uint64_t> key =
+                launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params);
             if (key != graph_key_) {
                 drop_graphs();
                 graph_key_ = key;

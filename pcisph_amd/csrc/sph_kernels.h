@@ -465,8 +465,10 @@ __global__ void k_fill_box(const SphParams<R>* __restrict__ scenes, const uint2*
 // fluid minimum (x, y, z) and maximum, then the boundary minimum and maximum, as
 // int32 folded with atomics (the caller sets minima to INT32_MAX, maxima to INT32_MIN).
 template <typename R>
-__global__ void k_extents(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
-                          const V4<R>* __restrict__ pos, const V4<R>* __restrict__ bpos, int* __restrict__ ext) {
+__global__ __launch_bounds__(kMaxBlockThreads) void k_extents(const SphParams<R>* __restrict__ scenes,
+                                                            const BlockRange* __restrict__ blocks,
+                                                            const V4<R>* __restrict__ pos,
+                                                            const V4<R>* __restrict__ bpos, int* __restrict__ ext) {
     const BlockRange rg = blocks[blockIdx.x];
     const uint32_t i = rg.begin + threadIdx.x;
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
@@ -477,6 +479,9 @@ __global__ void k_extents(const SphParams<R>* __restrict__ scenes, const BlockRa
         lo[1] = hi[1] = c.y;
         lo[2] = hi[2] = c.z;
     }
+    // wavefront, then workgroup reduction: one atomic per value and workgroup
+    __shared__ int s_ext[6][kMaxBlockThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -484,14 +489,19 @@ __global__ void k_extents(const SphParams<R>* __restrict__ scenes, const BlockRa
             lo[a] = min(lo[a], __shfl_xor(lo[a], off));
             hi[a] = max(hi[a], __shfl_xor(hi[a], off));
         }
-    }
-    if ((threadIdx.x & 63) == 0 && rg.begin + (threadIdx.x & ~63u) < rg.end) {
-        int* e = ext + 12 * rg.scene + (boundary ? 6 : 0);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(e + a, lo[a]);
-            atomicMax(e + 3 + a, hi[a]);
+        if (lane == 0) {
+            s_ext[a][wave] = lo[a];
+            s_ext[3 + a][wave] = hi[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int q = threadIdx.x;
+        int v = q < 3 ? INT32_MAX : INT32_MIN;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = q < 3 ? min(v, s_ext[q][w]) : max(v, s_ext[q][w]);
+        int* e = ext + 12 * rg.scene + (boundary ? 6 : 0) + q;
+        if (q < 3) atomicMin(e, v);
+        else atomicMax(e, v);
     }
 }
 
