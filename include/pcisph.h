@@ -97,8 +97,12 @@ typedef struct pcisph_solver_options {
                                        pass, where distances are taken at predicted positions,
                                        a wavefront walks lists only if for each of its
                                        particles its displacement plus the largest in its 27
-                                       buckets is at most 0.9 skin h, else its buckets.
-                                       0 = off; default 0.2 */
+                                       buckets is at most 0.9 skin h, else its buckets.  A
+                                       scene whose pressure passes walked their buckets in a
+                                       step (lists in under 1% of their wavefronts) builds
+                                       lists of radius 1.01 h (shorter lists for the passes at
+                                       x) until a probe step every 32 steps finds the full
+                                       radius used again.  0 = off; default 0.2 */
     uint32_t graph_iterations;      /* > 0: the step's launches are replayed from HIP graphs --
                                        one for everything before the correction loop and one
                                        per `graph_iterations` loop iterations (iterations past

@@ -479,6 +479,8 @@ class Engine final : public EngineBase {
         }
         blocks_for_ = 0;  // block table out of date
         boxes_dirty_ = true;
+        near_.assign(n.size(), 0);
+        near_steps_.assign(n.size(), 0);
         HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
@@ -800,9 +802,10 @@ class Engine final : public EngineBase {
             P.p_ref = (float)(P.delta * P.rho0 / R(100));
             P.div_scale = (float)((double)p.particle_mass * 45.0 /
                                   (3.14159265358979323846 * std::pow((double)P.h, 6)) * (double)P.dt * 100.0);
-            const float h = (float)P.h, radius = (1.f + list_skin_) * h;
-            P.list_r2 = list_skin_ > 0.f ? radius * radius : 0.f;
-            P.disp_limit = 0.9f * list_skin_ * h;
+            const float skin = scene_skin(s);
+            const float h = (float)P.h, radius = (1.f + skin) * h;
+            P.list_r2 = skin > 0.f ? radius * radius : 0.f;
+            P.disp_limit = 0.9f * skin * h;
             if (s < boxes_.size()) {
                 for (int a = 0; a < 3; ++a) {
                     P.box_lo[a] = boxes_[s].lo[a];
@@ -1177,6 +1180,39 @@ class Engine final : public EngineBase {
             r.warm_rejected = warm && q.warm_rejected;
             r.list_waves = q.list_waves;
         }
+        update_list_radius(out);
+    }
+
+    // The list radius of each scene.  Lists serve the passes at x whatever their
+    // radius, and the pressure passes only where particles stay within the skin; a
+    // scene whose pressure passes walked their buckets (lists in under 1% of their
+    // wavefronts) in a step builds lists of radius (1 + kNearSkin) h -- fewer entries
+    // to write and walk -- until a probe step every kProbeEvery steps with the full
+    // radius finds them used again.  Results do not depend on the radius.
+    static constexpr float kNearSkin = 0.01f;
+    static constexpr uint32_t kProbeEvery = 32;
+    float scene_skin(size_t s) const {
+        if (!(list_skin_ > 0.f)) return 0.f;
+        return s < near_.size() && near_[s] ? std::min(list_skin_, kNearSkin) : list_skin_;
+    }
+    void update_list_radius(const std::vector<StepResult>& out) {
+        near_.resize(out.size(), 0);
+        near_steps_.resize(out.size(), 0);
+        if (!A_.NL.on) return;
+        for (size_t s = 0; s < out.size(); ++s) {
+            if (!n_[s] || !out[s].iterations) continue;
+            const uint64_t waves =
+                ((uint64_t)(n_[s] + 63) / 64 + (uint64_t)(nb_[s] + 63) / 64) * (uint64_t)out[s].iterations;
+            const bool used = 100ull * out[s].list_waves >= waves;
+            if (!near_[s]) {
+                if (!used && list_skin_ > kNearSkin) {
+                    near_[s] = 1;
+                    near_steps_[s] = 0;
+                }
+            } else if (++near_steps_[s] >= kProbeEvery) {
+                near_[s] = 0;  // the next step probes the full radius
+            }
+        }
     }
 
     // Neighbour-list storage (capacities: fluid 128, boundary 96 per fluid particle,
@@ -1313,6 +1349,8 @@ class Engine final : public EngineBase {
     float* feat_ = nullptr;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
     float list_skin_ = 0.f;
+    std::vector<uint8_t> near_;          // per scene: lists of radius (1 + kNearSkin) h this step
+    std::vector<uint32_t> near_steps_;   // steps since the scene went to near lists
     uint32_t* okeys_[2] = {nullptr, nullptr};  // layout sort: bucket codes
     uint32_t* ovals_[2] = {nullptr, nullptr};  // and canonical indices (the result is the layout)
     uint32_t* canon_ = nullptr;                // slot -> canonical index of the public fluid arrays
