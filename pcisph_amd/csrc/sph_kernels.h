@@ -306,8 +306,10 @@ __device__ __forceinline__ void for_each_candidate(const uint2* __restrict__ cel
     });
 }
 
-// Per-step neighbour lists, slot-major (entry k of particle i at [k * stride + i], so
-// the lanes of a wavefront read consecutive words): for every fluid particle its
+// Per-step neighbour lists in chunks of four entries, slot-major: entries 4c..4c+3 of
+// particle i are the uint4 at [c * stride + i], so the lanes of a wavefront read
+// consecutive 16-byte chunks and a lane writes its list 16 bytes at a time: for
+// every fluid particle its
 // fluid and boundary candidates within the list radius, for every boundary particle
 // its fluid candidates, each in the reference's bucket order.  A particle with more
 // neighbours than the capacity gets kListOverflow and walks its buckets.
@@ -375,14 +377,27 @@ __device__ __forceinline__ float bucket_displacement(const uint32_t* __restrict_
     return __uint_as_float(m);
 }
 
+// Appends to particle i's list, four entries per store (the chunk's tail past the
+// list's length is left undefined).  `cap` is a multiple of 4.
 struct ListWriter {
     uint32_t* base;
     uint32_t stride, i, cap, len;
+    uint32_t e[4];
+    __device__ __forceinline__ void store_chunk() {
+        reinterpret_cast<uint4*>(base)[(size_t)(len >> 2) * stride + i] = make_uint4(e[0], e[1], e[2], e[3]);
+    }
     __device__ __forceinline__ void push(uint32_t j) {
-        if (len < cap) base[(size_t)len * stride + i] = j;
+        const uint32_t q = len & 3u;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) e[u] = q == u ? j : e[u];
+        if (q == 3u && len < cap) store_chunk();
         ++len;
     }
-    __device__ __forceinline__ uint32_t result() const { return len <= cap ? len : kListOverflow; }
+    // the last, partial chunk; then the length (kListOverflow past the capacity)
+    __device__ __forceinline__ uint32_t finish() {
+        if ((len & 3u) && len < cap) store_chunk();
+        return len <= cap ? len : kListOverflow;
+    }
 };
 
 
@@ -602,26 +617,37 @@ __device__ __forceinline__ void walk_buckets(const CellTable& T, int3 c, Load&& 
     for_each_range_row(T, c, [&](uint2 r0, uint2 r1, uint2 r2) { gather_row<U>(r0, r1, r2, load, use); });
 }
 
-// A slot-major list of `len` entries of particle i, U loads in flight per lane.
+// Particle i's list of `len` entries (chunks of four, slot-major), U loads in flight
+// per lane (U a multiple of 4).
 template <int U = kListBatch, typename Load, typename Use>
 __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
                                           uint32_t i, Load&& load, Use&& use) {
+    static_assert(U % 4 == 0, "list batches are whole chunks");
     using T = decltype(load(0u));
-    const uint32_t* q = list + i;
+    const uint4* q = reinterpret_cast<const uint4*>(list) + i;
     uint32_t k = 0;
     for (; k + U <= len; k += U) {
         uint32_t jj[U];
         T d[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = q[(size_t)(k + u) * stride];
+        for (int c = 0; c < U / 4; ++c) {
+            const uint4 e = q[(size_t)(k / 4 + c) * stride];
+            jj[4 * c] = e.x;
+            jj[4 * c + 1] = e.y;
+            jj[4 * c + 2] = e.z;
+            jj[4 * c + 3] = e.w;
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) d[u] = load(jj[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) use(d[u], jj[u]);
     }
-    for (; k < len; ++k) {
-        const uint32_t j = q[(size_t)k * stride];
-        use(load(j), j);
+    for (; k < len; k += 4) {
+        const uint4 e = q[(size_t)(k / 4) * stride];
+        const uint32_t jj[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < len) use(load(jj[u]), jj[u]);
     }
 }
 
@@ -661,7 +687,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
             const int3 c = cell_of4<R>(self, P.cell);
             R rho = 0;
             uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
-            ListWriter wb{NL.b, B.n_total, t, NL.cap_b, 0}, wf{NL.f, B.n_total, t, NL.cap_f, 0};
+            ListWriter wb{NL.b, B.n_total, t, NL.cap_b, 0, {}}, wf{NL.f, B.n_total, t, NL.cap_f, 0, {}};
             if (P.nb)
                 walk_buckets<R, kListBatch>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
                                 [&](const V4<R>& o, uint32_t j) {
@@ -687,8 +713,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
             xr[i] = out;
             if (counts) counts[i] = make_uint4(cf, cb, nf, nb);
             if (lists) {
-                NL.bcount[t] = P.nb ? wb.result() : 0u;
-                NL.fcount[t] = wf.result();
+                NL.bcount[t] = P.nb ? wb.finish() : 0u;
+                NL.fcount[t] = wf.finish();
             }
             cf_sum = cf;
             cb_sum = cb;
@@ -701,13 +727,13 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
         bxf[b] = o;
         if (lists) {
             const int3 c = cell_of4<R>(o, P.cell);
-            ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0};
+            ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0, {}};
             walk_buckets<R, kListBatch>(FT, c, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& q, uint32_t j) {
                                 R dx, dy, dz;
                                 if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
                             });
-            NL.bfcount[b] = w.result();
+            NL.bfcount[b] = w.finish();
         }
     }
     const unsigned long long a = block_reduce_sum(cf_sum, lds);
