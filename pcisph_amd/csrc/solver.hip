@@ -805,7 +805,8 @@ class Engine final : public EngineBase {
             const float skin = scene_skin(s);
             const float h = (float)P.h, radius = (1.f + skin) * h;
             P.list_r2 = skin > 0.f ? radius * radius : 0.f;
-            P.disp_limit = 0.9f * skin * h;
+            // near lists serve the passes at x only: no displacement tracking or tests
+            P.disp_limit = s < near_.size() && near_[s] ? 0.f : 0.9f * skin * h;
             if (s < boxes_.size()) {
                 for (int a = 0; a < 3; ++a) {
                     P.box_lo[a] = boxes_[s].lo[a];
@@ -1187,7 +1188,8 @@ class Engine final : public EngineBase {
     // radius, and the pressure passes only where particles stay within the skin; a
     // scene whose pressure passes walked their buckets (lists in under 1% of their
     // wavefronts) in a step builds lists of radius (1 + kNearSkin) h -- fewer entries
-    // to write and walk -- until a probe step every kProbeEvery steps with the full
+    // to write and walk -- and neither tracks displacements nor tests its lists in
+    // the pressure passes, until a probe step every kProbeEvery steps with the full
     // radius finds them used again.  Results do not depend on the radius.
     static constexpr float kNearSkin = 0.01f;
     static constexpr uint32_t kProbeEvery = 32;

@@ -889,7 +889,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, con
                                  a_div * P.div_scale);
         }
     }
-    if (NL.on) record_displacement(NL.disp[0], bucket, moved);
+    if (NL.on && P.disp_limit > 0.f) record_displacement(NL.disp[0], bucket, moved);
 }
 
 // update_pressure (pcisph.cl:212-263): fluid mode on fluid ranges, boundary mode on
@@ -921,7 +921,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
     // fewer than 1% of whose wavefronts could use them in the step's passes so far
     // stops testing from the third pass on (displacements hardly change within a step)
     const uint32_t waves = (P.n + 63) / 64 + (P.nb + 63) / 64;
-    const bool try_lists = NL.on && (iter < 2 || 100ull * st->list_waves >= (unsigned long long)waves * iter);
+    const bool try_lists = NL.on && P.disp_limit > 0.f &&
+                           (iter < 2 || 100ull * st->list_waves >= (unsigned long long)waves * iter);
     bool ok = true;
     V4<R> self{}, pr{};
     int3 c{};
@@ -1321,7 +1322,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
             if (bad) atomicOr(&st->nonfinite, 1u);
         }
     }
-    if (!last && !warm_pass && NL.on) record_displacement(NL.disp[(iter + 1) & 1], bucket, moved);
+    if (!last && !warm_pass && NL.on && P.disp_limit > 0.f)
+        record_displacement(NL.disp[(iter + 1) & 1], bucket, moved);
 }
 
 // The network's start: p = max(0, p_prev + y p_ref), y the inference kernel's output
