@@ -44,10 +44,10 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
     for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d * tiles + blockIdx.x] = s_hist[d];
 }
 
-// Exclusive scan of `count` words in three launches: chunks of kScanChunk words
-// scanned by one workgroup each (k_scan_chunks, chunk totals to `partials`), the
-// totals scanned by one workgroup (k_scan_totals), the totals added back
-// (k_scan_add).
+// Exclusive scan of `count` words: chunks of kScanChunk words scanned by one
+// workgroup each (k_scan_chunks, chunk totals to `partials`), the totals scanned --
+// by one workgroup (k_scan_totals) when there are at most 1024 of them, else by the
+// same scan one level up -- and added back (k_scan_add).
 constexpr int kScanThreads = 256;
 constexpr int kScanPer = 8;
 constexpr int kScanChunk = kScanThreads * kScanPer;
@@ -193,10 +193,27 @@ void fill_u64(const Launch& L, unsigned long long* p, size_t n, unsigned long lo
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, L.stream, p, n, value);
 }
 
+// Words of chunk totals the scan of `count` words needs at every level.
+size_t scan_partial_words(size_t count) {
+    size_t words = 0;
+    for (size_t c = (count + kScanChunk - 1) / kScanChunk; c > 1; c = (c + kScanChunk - 1) / kScanChunk)
+        words += c;
+    return words + 1024;
+}
+
+void exclusive_scan(const Launch& L, uint32_t* data, uint32_t count, uint32_t* partials) {
+    const uint32_t chunks = (count + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(kScanThreads), 0, L.stream, data, count, partials);
+    if (chunks <= 1) return;
+    if (chunks <= 1024) hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, L.stream, partials, chunks);
+    else exclusive_scan(L, partials, chunks, partials + chunks);
+    hipLaunchKernelGGL(k_scan_add, dim3(chunks), dim3(kScanThreads), 0, L.stream, data, count, partials);
+}
+
 size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits) {
     const size_t tiles = (n + kTile - 1) / kTile;
     const size_t words = (size_t(1) << digit_bits) * (tiles ? tiles : 1);
-    return words + 1024;  // + the chunk totals of the scan
+    return words + scan_partial_words(words);  // histograms, then the scan's chunk totals
 }
 
 int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
@@ -209,12 +226,7 @@ int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n
         const uint32_t words = (1u << bits) * tiles;
         hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], n, shift, bits,
                            scratch, tiles);
-        const uint32_t chunks = (words + kScanChunk - 1) / kScanChunk;
-        if (chunks > 1024) return -1;  // more than 2^24 keys at 8-bit digits
-        uint32_t* partials = scratch + (radix_sort_scratch_words(n, digit_bits) - 1024);
-        hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(kScanThreads), 0, L.stream, scratch, words, partials);
-        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, L.stream, partials, chunks);
-        hipLaunchKernelGGL(k_scan_add, dim3(chunks), dim3(kScanThreads), 0, L.stream, scratch, words, partials);
+        exclusive_scan(L, scratch, words, scratch + (size_t(1) << digit_bits) * tiles);
         hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], vals[cur],
                            keys[cur ^ 1], vals[cur ^ 1], n, shift, bits, scratch, tiles);
         cur ^= 1;

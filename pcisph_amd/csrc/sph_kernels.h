@@ -564,7 +564,8 @@ __device__ __forceinline__ float displacement(const V4<R>& a, const V4<R>& b) {
 // use(data, j): a batch of candidates is loaded before any is used, so each lane
 // keeps that many independent loads in flight instead of one; use() still sees the
 // candidates in order, so sums are unchanged.  Measured best: 4 for bucket ranges
-// (short, ~8 candidates), 8 for neighbour lists (index loads first, then data).
+// (every pass that walks buckets, the list-building density pass included), 8 for
+// neighbour lists (index chunks first, then data).
 constexpr int kGatherBatch = 4;
 constexpr int kListBatch = 8;
 constexpr int kForceBatch = 8;  // force init: three float4 per candidate
@@ -586,8 +587,7 @@ __device__ __forceinline__ void gather_range(uint32_t start, uint32_t end, Load&
 // The three ranges of a bucket row gathered as one sequence (r0, then r1, then r2):
 // batches of U loads run across bucket boundaries, so a row of ~24 candidates is a
 // few full batches instead of three short ranges each ending in a remainder loop
-// (measured: the pressure pass 466 -> 362 us at 1M with U = 4; the density pass
-// prefers U = 8).
+// (measured: the pressure pass 466 -> 362 us at 1M with U = 4).
 template <int U, typename Load, typename Use>
 __device__ __forceinline__ void gather_row(uint2 r0, uint2 r1, uint2 r2, Load&& load, Use&& use) {
     using T = decltype(load(0u));
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
             uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
             ListWriter wb{NL.b, B.n_total, t, NL.cap_b, 0, {}}, wf{NL.f, B.n_total, t, NL.cap_f, 0, {}};
             if (P.nb)
-                walk_buckets<R, kListBatch>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
+                walk_buckets<R>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
                                 [&](const V4<R>& o, uint32_t j) {
                                     R dx, dy, dz;
                                     const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
                                     nb += !(r2 > P.h2);
                                     if (lists && r2 <= lr2) wb.push(j);
                                 });
-            walk_buckets<R, kListBatch>(FT, c, [&](uint32_t j) { return pos[j]; },
+            walk_buckets<R>(FT, c, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& o, uint32_t j) {
                                 R dx, dy, dz;
                                 const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
@@ -725,10 +725,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density(Batch<R> B, const 
         V4<R> o = bpos[b];
         o.w = 0;
         bxf[b] = o;
-        if (lists) {
+        if (lists && P.disp_limit > 0.f) {  // read only by pressure passes that test their lists
             const int3 c = cell_of4<R>(o, P.cell);
             ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0, {}};
-            walk_buckets<R, kListBatch>(FT, c, [&](uint32_t j) { return pos[j]; },
+            walk_buckets<R>(FT, c, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& q, uint32_t j) {
                                 R dx, dy, dz;
                                 if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);

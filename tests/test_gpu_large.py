@@ -75,3 +75,38 @@ def test_64k_ieee_matches_oracle(oracle_lib):
                 Field.BOUNDARY_SRC_LOCATIONS, Field.FLUID_DENSITIES, Field.FLUID_POSITIONS,
                 Field.FLUID_VELOCITIES, Field.FLUID_PRESSURES):
         np.testing.assert_array_equal(g.download(fld), o.download(fld), err_msg=Field(fld).name)
+
+
+def test_sort_beyond_2p24_keys():
+    """2^24 + 2^20 fluid particles: the radix sort's histogram scan has more chunk
+    totals than one workgroup scans and recurses.  The sort products must still be the
+    stable sort of the hash keys (checked through properties: sorted keys, a
+    permutation, keys[c] = hash(src[c]), ascending sources within equal keys, the cell
+    table at every key transition)."""
+    import pcisph_amd
+    nx, ny, nz = 256, 272, 256
+    n = nx * ny * nz
+    assert n == (1 << 24) + (1 << 20)
+    rng = np.random.default_rng(11)
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    pos = (g.astype(np.float32) * np.float32(0.02) + np.float32(0.05) +
+           rng.uniform(-0.002, 0.002, g.shape).astype(np.float32)).astype(np.float32).ravel()
+    del g
+    scene = scenes.Scene(name="block_17m", fluid=pos, boundary=np.zeros(0, np.float32), particle_radius=0.01)
+    print(f"case sort beyond 2^24 N={n}", flush=True)
+    f = pcisph_amd.Fluid()
+    f.load_scene(scene, max_iterations=3, neighbor_skin=0.0)
+    f.step()
+    p = f.get_params()
+    keys, src = f.download(Field.FLUID_KEYS), f.download(Field.FLUID_SRC_LOCATIONS)
+    assert np.all(keys[1:] >= keys[:-1]), "keys not sorted"
+    assert np.array_equal(np.sort(src), np.arange(n, dtype=np.uint32)), "src is not a permutation"
+    np.testing.assert_array_equal(keys, np_hash_keys(pos, p.cell_size, p.bucket_count)[src])
+    same = keys[1:] == keys[:-1]
+    assert np.all(src[1:][same] > src[:-1][same]), "sort is not stable"
+    cells = f.download(Field.FLUID_CELL_OFFSETS).reshape(-1, 2)
+    starts = np.nonzero(np.concatenate([[True], ~same]))[0]
+    ends = np.concatenate([starts[1:], [n]])
+    np.testing.assert_array_equal(cells[keys[starts], 0], starts)
+    np.testing.assert_array_equal(cells[keys[starts], 1], ends)
+    f.close()
