@@ -103,9 +103,11 @@ def roofline_of(times: dict, n: int, cf: float, cb: float, traffic: bool = False
 
 
 def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, traffic: bool = False,
-            scene=None) -> dict:
+            scene=None, profile_every: int = 8) -> dict:
     """`warmup` untimed then `steps` timed steps (every scene of a batch), HIP-event
-    stage times over the timed region when `profile`.  Moving walls (sloshing scenes)
+    stage times over the timed region when `profile`: every `profile_every`-th step of
+    it is launched kernel by kernel between events, the others replay their HIP graphs
+    (events cannot be read back from graph replays).  Moving walls (sloshing scenes)
     are translated before every step."""
     from pcisph_amd import scenes
     shift = None
@@ -130,7 +132,7 @@ def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, t
     fluid.sync()
     lat.clear()
     if profile:
-        fluid.profile_enable(True)
+        fluid.profile_enable(True, every=profile_every)
     t0 = time.perf_counter()
     stats = go(steps, warmup)
     fluid.sync()

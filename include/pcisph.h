@@ -109,8 +109,8 @@ typedef struct pcisph_solver_options {
                                        the end of every scene's loop return at once on the
                                        device).  Graphs are captured on first use and again
                                        whenever a launch argument changes; results are
-                                       identical to 0 (launch by launch).  Default 4.  Stage
-                                       profiling launches eagerly whatever this says. */
+                                       identical to 0 (launch by launch).  Default 4.  Steps
+                                       timed by the stage profiler launch kernel by kernel. */
     uint32_t work_order;            /* bit 0: fluid particles are stored in a Morton layout
                                        rebuilt every step -- each bucket whole and in the
                                        reference's order, the buckets along a Morton curve of
@@ -249,6 +249,8 @@ typedef struct pcisph_stage_times {
     uint64_t launches[16];/* timed launches per stage */
 } pcisph_stage_times;
 
+/* enable = 0: off; 1: time every step; n > 1: time every n-th step (the others replay
+ * their HIP graphs untimed).  Timed steps launch kernel by kernel. */
 int pcisph_profile_enable(pcisph_solver* s, int enable);
 /* Waits for the stream, then returns the totals since profiling was enabled. */
 int pcisph_profile_read(pcisph_solver* s, pcisph_stage_times* out);

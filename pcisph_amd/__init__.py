@@ -164,8 +164,10 @@ class Fluid(SolverBase):
         """Apply pending setter changes now (see pcisph_prepare)."""
         self._call("prepare")
 
-    def profile_enable(self, on: bool = True):
-        self._call("profile_enable", C.c_int(1 if on else 0))
+    def profile_enable(self, on=True, every: int = 1):
+        """Stage timing with HIP events: off, or every `every`-th step (the others replay
+        their graphs untimed)."""
+        self._call("profile_enable", C.c_int(max(1, int(every)) if on else 0))
 
     def profile_read(self) -> dict:
         t = StageTimes()

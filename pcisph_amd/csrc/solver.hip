@@ -127,7 +127,10 @@ uint32_t key_bits_for(uint32_t buckets) {
 
 // HIP-event timing of the step's stages.  Event pairs are recorded around each
 // stage's launches and resolved lazily (at read time, or when many are pending at
-// the end of a step), so timing never waits on the stream inside a step.
+// the end of a step), so timing never waits on the stream inside a step.  Timed
+// steps launch kernel by kernel (events cannot be read back from HIP-graph
+// replays); with a sampling interval n > 1 only every n-th step is timed and the
+// others replay their graphs.
 class StageProfiler {
    public:
     ~StageProfiler() {
@@ -135,17 +138,26 @@ class StageProfiler {
         for (hipEvent_t e : free_) (void)hipEventDestroy(e);
     }
 
-    void enable(bool on, hipStream_t s) {
-        if (!on_ && !on) return;
+    // every = 0: off; n: time every n-th step from the next one
+    void enable(uint32_t every, hipStream_t s) {
+        if (!on_ && !every) return;
         resolve(s);
-        on_ = on;
+        on_ = every != 0;
+        every_ = every ? every : 1;
+        first_ = step_;
         for (auto& v : ms_) v = 0;
         for (auto& v : launches_) v = 0;
     }
 
+    // Whether the step about to run is timed.
+    bool begin_step() {
+        active_ = on_ && (step_ - first_) % every_ == 0;
+        return active_;
+    }
+
     template <typename F>
     void timed(int stage, hipStream_t s, int iter, F&& fn) {
-        if (!on_) {
+        if (!active_) {
             fn();
             return;
         }
@@ -158,9 +170,10 @@ class StageProfiler {
 
     // Iterations after `last_iter` of this step were enqueued ahead and skipped.
     void end_step(int last_iter) {
-        if (!on_) return;
-        for (auto& p : pending_)
-            if (p.step == step_ && p.iter > last_iter) p.skipped = true;
+        if (active_)
+            for (auto& p : pending_)
+                if (p.step == step_ && p.iter > last_iter) p.skipped = true;
+        active_ = false;
         ++step_;
     }
 
@@ -214,8 +227,9 @@ class StageProfiler {
         pending_.clear();
     }
 
-    bool on_ = false;
-    uint64_t step_ = 0;
+    bool on_ = false, active_ = false;
+    uint64_t step_ = 0, first_ = 0;
+    uint32_t every_ = 1;
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> free_;
     double ms_[PCISPH_STAGE_COUNT] = {};
@@ -264,7 +278,7 @@ class EngineBase {
                       bool fast, const WarmArgs& w, std::vector<StepResult>& out, uint64_t* samples) = 0;
     virtual std::vector<uint32_t> nonfinite() = 0;
     virtual size_t field_size(uint32_t scene, int field, const Simulation_Params& p) const = 0;
-    virtual void profile_enable(bool on) = 0;
+    virtual void profile_enable(uint32_t every) = 0;
     virtual void profile_read(pcisph_stage_times* out) = 0;
     virtual void translate_boundary(uint32_t scene, float dx, float dy, float dz) = 0;
     // every scene's walls at once: shifts[3 s .. 3 s + 2] for scene s
@@ -718,7 +732,7 @@ class Engine final : public EngineBase {
         HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
-    void profile_enable(bool on) override { prof_.enable(on, stream_); }
+    void profile_enable(uint32_t every) override { prof_.enable(every, stream_); }
     void profile_read(pcisph_stage_times* out) override { prof_.read(stream_, out); }
 
     std::vector<uint32_t> nonfinite() override {
@@ -1003,8 +1017,9 @@ class Engine final : public EngineBase {
         D_.disp_next = A_.NL.on ? A_.NL.disp[1] : nullptr;
         D_.buckets = A_.NL.buckets;
         if (!ctl_) dalloc(ctl_, 64, stream_);
-        // graphs record launches; the stage profiler times them one by one
-        const uint32_t chunk = prof_.on() ? 0u : o.graph_iterations;
+        // graphs record launches; a timed step launches them one by one (and keeps the
+        // graphs for the untimed ones)
+        const uint32_t chunk = prof_.begin_step() ? 0u : o.graph_iterations;
         if (chunk) {
             std::vector<uint64_t> key =
                 launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params);
@@ -1012,7 +1027,7 @@ class Engine final : public EngineBase {
                 drop_graphs();
                 graph_key_ = key;
             }
-        } else if (!graph_key_.empty()) {
+        } else if (!o.graph_iterations && !graph_key_.empty()) {
             drop_graphs();
         }
         // this step's loop records and tag (outside any graph: the tag changes per step)
@@ -2063,7 +2078,10 @@ int pcisph_restore(pcisph_solver* s) {
 }
 
 int pcisph_profile_enable(pcisph_solver* s, int enable) {
-    return guarded(s, [&] { s->eng->profile_enable(enable != 0); });
+    return guarded(s, [&] {
+        if (enable < 0) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "negative profiling interval");
+        s->eng->profile_enable((uint32_t)enable);
+    });
 }
 
 int pcisph_profile_read(pcisph_solver* s, pcisph_stage_times* out) {
