@@ -99,7 +99,7 @@ typedef struct pcisph_solver_options {
                                        particles its displacement plus the largest in its 27
                                        buckets is at most 0.9 skin h, else its buckets.  A
                                        scene whose pressure passes walked their buckets in a
-                                       step (lists in under 1% of their wavefronts) builds
+                                       step (lists in under 1% of their fluid wavefronts) builds
                                        lists of radius 1.01 h (shorter lists for the passes at
                                        x) until a probe step every 32 steps finds the full
                                        radius used again.  0 = off; default 0.2 */
@@ -142,9 +142,9 @@ typedef struct pcisph_step_stats {
                                        step-start density was >= rho0 (free-surface particles
                                        sit below rho0 regardless), last iteration: the side of
                                        the residual the loop does not bound */
-    uint32_t list_waves;            /* wavefronts (64 particles) of this step's pressure passes
-                                       that walked neighbour lists; the others walked their
-                                       27 buckets (sum over the step's iterations) */
+    uint32_t list_waves;            /* fluid wavefronts (64 particles) of this step's pressure
+                                       passes that walked neighbour lists; the others walked
+                                       their 27 buckets (sum over the step's iterations) */
 } pcisph_step_stats;
 
 typedef struct pcisph_solver pcisph_solver;

@@ -1202,7 +1202,7 @@ class Engine final : public EngineBase {
     // The list radius of each scene.  Lists serve the passes at x whatever their
     // radius, and the pressure passes only where particles stay within the skin; a
     // scene whose pressure passes walked their buckets (lists in under 1% of their
-    // wavefronts) in a step builds lists of radius (1 + kNearSkin) h -- fewer entries
+    // fluid wavefronts) in a step builds lists of radius (1 + kNearSkin) h -- fewer entries
     // to write and walk -- and neither tracks displacements nor tests its lists in
     // the pressure passes, until a probe step every kProbeEvery steps with the full
     // radius finds them used again.  Results do not depend on the radius.
@@ -1218,8 +1218,7 @@ class Engine final : public EngineBase {
         if (!A_.NL.on) return;
         for (size_t s = 0; s < out.size(); ++s) {
             if (!n_[s] || !out[s].iterations) continue;
-            const uint64_t waves =
-                ((uint64_t)(n_[s] + 63) / 64 + (uint64_t)(nb_[s] + 63) / 64) * (uint64_t)out[s].iterations;
+            const uint64_t waves = (uint64_t)(n_[s] + 63) / 64 * (uint64_t)out[s].iterations;
             const bool used = 100ull * out[s].list_waves >= waves;
             if (!near_[s]) {
                 if (!used && list_skin_ > kNearSkin) {

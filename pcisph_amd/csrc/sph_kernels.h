@@ -98,7 +98,7 @@ struct LoopState {
     unsigned long long dual_exp[2];
     unsigned int nonfinite;
     unsigned int warm_rejected;  // 1: the plain start had the smaller residual
-    unsigned int list_waves;     // wavefronts of the pressure passes that walked neighbour lists
+    unsigned int list_waves;     // fluid wavefronts of the pressure passes that walked neighbour lists
     unsigned int pad;
 };
 
@@ -918,9 +918,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
     const bool boundary = (rg.flags & kRangeBoundary) != 0;
     const uint32_t i = t;
     // lists for this wavefront if every lane's pairs are all in its list; a scene
-    // fewer than 1% of whose wavefronts could use them in the step's passes so far
-    // stops testing from the third pass on (displacements hardly change within a step)
-    const uint32_t waves = (P.n + 63) / 64 + (P.nb + 63) / 64;
+    // fewer than 1% of whose fluid wavefronts could use them in the step's passes so
+    // far stops testing from the third pass on (displacements hardly change within a
+    // step; boundary wavefronts far from the fluid pass trivially and are not counted)
+    const uint32_t waves = (P.n + 63) / 64;
     const bool try_lists = NL.on && P.disp_limit > 0.f &&
                            (iter < 2 || 100ull * st->list_waves >= (unsigned long long)waves * iter);
     bool ok = true;
@@ -937,7 +938,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
     }
     const bool lists = try_lists && __all(ok);
     __syncthreads();
-    if (lists && (threadIdx.x & 63) == 0) atomicAdd(&s_list_waves, 1u);
+    if (lists && !boundary && (threadIdx.x & 63) == 0) atomicAdd(&s_list_waves, 1u);
     unsigned long long m[2] = {0, 0};  // max dv, max expansion
     if (!boundary) {
         if (t < rg.end) {

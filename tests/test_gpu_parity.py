@@ -237,6 +237,52 @@ def test_list_overflow_falls_back_to_buckets(gpu, oracle_lib):
         np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
 
 
+def test_list_radius_adapts_bit_exact(gpu, oracle_lib):
+    """Particles thrown at 10 m/s move further per step than a 0.05 h skin allows, so
+    the pressure passes walk their buckets; the scene then builds near-h lists with no
+    displacement tracking, and every 32 steps probes the full skin.  Results stay the
+    oracle's bit for bit through both switches."""
+    scene = small_scenes()[1]
+    print("case list radius adapts", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    o.load_scene(scene, max_iterations=20)
+    g.load_scene(scene, max_iterations=20, neighbor_skin=0.05)
+    v = np.tile(np.array([0.0, -10.0, 0.0], np.float32), scene.fluid_count)
+    o.upload(Field.FLUID_VELOCITIES, v)
+    g.upload(Field.FLUID_VELOCITIES, v)
+    waves = []
+    for _ in range(35):
+        a, b = o.step(), g.step()
+        assert a["iterations"] == b["iterations"]
+        assert a["max_density_variation"] == b["max_density_variation"]
+        waves.append(b["list_waves"])
+    # steps 2..33 on near lists: no list tests in the pressure passes
+    assert all(w == 0 for w in waves[1:33]), waves
+    ro, rg = snapshot(o), snapshot(g)
+    for name in ro:
+        np.testing.assert_array_equal(rg[name], ro[name], err_msg=name)
+
+
+def test_sampled_stage_profile(gpu):
+    """Stage timing of every n-th step: a timed step launches kernel by kernel, the
+    others replay their graphs; the density pass runs once per step."""
+    scene = small_scenes()[1]
+    print("case sampled stage profile", flush=True)
+    g = gpu.Fluid()
+    g.load_scene(scene, max_iterations=20)
+    g.step()
+    for every, steps, timed in ((1, 4, 4), (3, 7, 3)):
+        g.profile_enable(True, every=every)
+        for _ in range(steps):
+            g.step()
+        t = g.profile_read()
+        g.profile_enable(False)
+        density = 4  # PCISPH_STAGE_DENSITY
+        assert t["launches"][density] == timed, (every, t["launches"])
+        assert t["ms"][density] > 0
+
+
 @pytest.mark.parametrize("work_order", [0, 2, 4, 7])
 def test_work_orders_are_exact(gpu, oracle_lib, work_order):
     """Every work order (reference slot order, XCD placement, LDS-staged pressure
