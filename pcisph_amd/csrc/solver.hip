@@ -1002,7 +1002,7 @@ class Engine final : public EngineBase {
             rebuild_boxes(L, S);
             prepare_scenes(scenes, o);
         }
-        // work order: Morton codes with `obits` bits per axis below the scene index
+        // Morton layout: bucket codes with `obits` bits per axis below the scene index
         const uint32_t sbits = key_bits_for(S);
         const uint32_t obits = std::min(7u, sbits < 32 ? (32u - sbits) / 3u : 0u);
         const bool layout = (o.work_order & 1) && obits >= 2 && A_.n > 1;

@@ -285,8 +285,9 @@ def test_sampled_stage_profile(gpu):
 
 @pytest.mark.parametrize("work_order", [0, 2, 4, 7])
 def test_work_orders_are_exact(gpu, oracle_lib, work_order):
-    """Every work order (reference slot order, XCD placement, LDS-staged pressure
-    passes, all together) leaves the oracle's bits."""
+    """Every storage order and placement (the reference's order, XCD placement,
+    LDS-staged pressure passes, all together with the Morton layout) leaves the
+    oracle's bits."""
     scene = small_scenes()[1]
     print(f"case work_order={work_order}", flush=True)
     o = oracle_lib.OracleFluid()
