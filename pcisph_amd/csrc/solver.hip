@@ -261,6 +261,7 @@ struct WarmArgs {
 struct SceneInput {
     const Simulation_Params* params;
     float threshold;
+    bool warm = true;  // the warm start applies to this scene (when the step uses it)
 };
 
 class EngineBase {
@@ -828,6 +829,7 @@ class Engine final : public EngineBase {
                 }
                 P.box_base = boxes_[s].base;
             }
+            P.warm = scenes[s].warm ? 1u : 0u;
             t[s] = P;
         }
         if (t.size() == scene_host_.size() &&
@@ -1192,8 +1194,8 @@ class Engine final : public EngineBase {
             r.converged = (r.iterations >= minit) && (mdv / rho0 < scene_host_[s].threshold);
             r.fluid_candidates = q.candidates[0];
             r.boundary_candidates = q.candidates[1];
-            r.warm = warm;
-            r.warm_rejected = warm && q.warm_rejected;
+            r.warm = warm && scenes[s].warm;
+            r.warm_rejected = r.warm && q.warm_rejected;
             r.list_waves = q.list_waves;
         }
         update_list_radius(out);
@@ -1434,6 +1436,7 @@ struct pcisph_solver {
         bool params_changed = true;
         bool boundary_updated = true;
         float threshold = 0.01f;
+        bool capped = false;  // its last step ran to the iteration cap without converging
     };
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1756,8 +1759,14 @@ static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
     apply_param_changes(s);
     if (out) out->assign(s->scenes.size(), pcisph_step_stats{});
     if (!s->total_fluid()) return;
+    // The warm start is left out for a scene whose last step ran to the iteration cap
+    // without converging (a start cannot shorten a loop the cap ends; the scene's step
+    // is then exactly the plain one), and out of the step when that holds for every
+    // scene (no inference, network pressure force or dual first iteration).
     WarmArgs w;
-    if (s->warm_start && s->mlp) w.params = s->mlp;
+    bool any_warm = false;
+    for (const auto& sc : s->scenes) any_warm = any_warm || (sc.params.fluid_count && !sc.capped);
+    if (s->warm_start && s->mlp && any_warm) w.params = s->mlp;
     if (s->collect_stride && s->samples_x && s->samples_count < s->samples_cap) {
         w.collect_stride = s->collect_stride;
         w.collect_phase = s->collect_phase++;
@@ -1769,7 +1778,7 @@ static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
     std::vector<pcisph::SceneInput> in;
     bool sort_boundary = s->opt.resort_boundary != 0;
     for (auto& sc : s->scenes) {
-        in.push_back({&sc.params, sc.threshold});
+        in.push_back({&sc.params, sc.threshold, !sc.capped});
         sort_boundary = sort_boundary || sc.boundary_updated;
     }
     std::vector<StepResult> res;
@@ -1777,6 +1786,8 @@ static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
     s->eng->step(in, s->opt, sort_boundary, s->precision == kPrecisionFast, w, res, &samples);
     s->samples_count = std::min(s->samples_cap, s->samples_count + samples);
     for (auto& sc : s->scenes) sc.boundary_updated = false;
+    for (size_t i = 0; i < res.size() && i < s->scenes.size(); ++i)
+        s->scenes[i].capped = res[i].iterations >= s->opt.max_iterations && !res[i].converged;
     if (out)
         for (size_t i = 0; i < res.size(); ++i) (*out)[i] = to_stats(s, res[i], s->scenes[i].params);
 }
@@ -2010,6 +2021,7 @@ int pcisph_set_warm_start(pcisph_solver* s, int enable) {
     return guarded(s, [&] {
         if (enable) s->ensure_mlp();
         s->warm_start = enable != 0;
+        for (auto& sc : s->scenes) sc.capped = false;
     });
 }
 

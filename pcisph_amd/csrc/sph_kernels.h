@@ -75,6 +75,7 @@ struct SphParams {
     uint32_t box_lo[3];              // cell box of the dense range grids (CellTable); box_dim[0] == 0: none
     uint32_t box_dim[3];
     uint32_t box_base;               // its first entry in the batch's grids
+    uint32_t warm;                   // 1: a warm-started step starts this scene from the network
 };
 
 // One workgroup's share of a launch: particles [begin, end) of one scene.
@@ -1337,8 +1338,10 @@ __global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, const V4<
     const BlockRange rg = B.blocks[blockIdx.x];
     const uint32_t i = rg.begin + threadIdx.x;
     if (i >= rg.end) return;
-    const float p_ref = B.scenes[rg.scene].p_ref;
-    const R pi = fmax(R(0), vel[i].w + (R)((float)p[i] * p_ref));
+    const SphParams<R>& P = B.scenes[rg.scene];
+    // a scene left out of the warm start starts from zero: its network pressure force
+    // is then exactly zero and its step exactly the plain one
+    const R pi = P.warm ? fmax(R(0), vel[i].w + (R)((float)p[i] * P.p_ref)) : R(0);
     p[i] = pi;
     V4<R> o = xr[i];
     o.w = pi / (o.w * o.w);
