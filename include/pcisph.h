@@ -86,8 +86,10 @@ typedef struct pcisph_solver_options {
     uint32_t resort_boundary;       /* 1: re-sort boundary particles every step (moving walls).
                                        The reference sorts them only after a parameter change
                                        (Fluid.cpp:135,173). */
-    uint32_t block_size;            /* threads per block of the per-particle kernels;
-                                       0 = autotuned/default */
+    uint32_t block_size;            /* threads per block of the per-particle kernels (a
+                                       multiple of 64, at most 512); 0 = by size: 256 from
+                                       65,536 fluid particles in the batch, 128 from 32,768,
+                                       else 64 */
     uint32_t sort_digit_bits;       /* radix digit width, 4..8; 0 = autotuned/default */
     float neighbor_skin;            /* neighbour lists: each step the density pass records, in
                                        the reference's summation order, every candidate within
@@ -111,7 +113,8 @@ typedef struct pcisph_solver_options {
                                        whenever a launch argument changes; results are
                                        identical to 0 (launch by launch).  Default 4.  Steps
                                        timed by the stage profiler launch kernel by kernel. */
-    uint32_t work_order;            /* bit 0: fluid particles are stored in a Morton layout
+    uint32_t work_order;            /* bit 0: fluid particles of batches of at least 131,072
+                                       are stored in a Morton layout
                                        rebuilt every step -- each bucket whole and in the
                                        reference's order, the buckets along a Morton curve of
                                        their cells -- so a wavefront covers a compact block of
@@ -122,8 +125,9 @@ typedef struct pcisph_solver_options {
                                        compact chunks of that order; bit 2: FP32 pressure
                                        passes with 256-thread workgroups stage each
                                        workgroup's neighbour buckets in LDS and walk them
-                                       there (measured slower on the dam breaks).
-                                       Default 3 (bits 0 and 1). */
+                                       there (measured slower on the dam breaks); bit 3: the
+                                       Morton layout (bit 0) at any size.  Default 3 (bits 0
+                                       and 1). */
     uint32_t reserved;
 } pcisph_solver_options;
 

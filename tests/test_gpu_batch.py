@@ -76,6 +76,20 @@ def test_batch_equals_scenes_alone(gpu, precision):
                 np.testing.assert_array_equal(g.download_f64(f), alone.download_f64(f), err_msg=Field(f).name)
 
 
+def test_batch_equals_scenes_alone_with_layout(gpu):
+    """The same with the Morton layout applied at these sizes (work_order bit 3): a
+    scene's buckets are laid out within its own range, so batch and alone still agree
+    bit for bit."""
+    batch = mixed_scenes()
+    print("case batch_equals_alone with layout", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    sb = run_batch(g, batch, 3, max_iterations=20, work_order=3 | 8)
+    for k, sc in enumerate(batch):
+        alone = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+        sa = run(alone, sc, 3, max_iterations=20, work_order=3 | 8)
+        compare_scene(g, k, alone, sb, sa, FLOAT_FIELDS + INT_FIELDS)
+
+
 def test_batch_without_lists_and_with_empty_scene(gpu):
     a, b = small_scenes()
     empty = scenes.Scene(name="empty", fluid=np.zeros(0, np.float32), boundary=np.zeros(0, np.float32),

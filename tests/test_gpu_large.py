@@ -68,7 +68,7 @@ def test_64k_ieee_matches_oracle(oracle_lib):
     o = oracle_lib.OracleFluid()
     o.load_scene(scene, max_iterations=10)
     g = pcisph_amd.Fluid(precision=pcisph_amd.PRECISION_IEEE)
-    g.load_scene(scene, max_iterations=10)
+    g.load_scene(scene, max_iterations=10, work_order=3 | 8)  # the Morton layout at this size
     a, b = o.step(), g.step()
     assert a["iterations"] == b["iterations"]
     for fld in (Field.FLUID_KEYS, Field.FLUID_SRC_LOCATIONS, Field.FLUID_CELL_OFFSETS,

@@ -283,7 +283,9 @@ def test_sampled_stage_profile(gpu):
         assert t["ms"][density] > 0
 
 
-@pytest.mark.parametrize("work_order", [0, 2, 4, 7])
+# bit 3 applies the Morton layout (bit 0) below its size threshold, as these small
+# scenes need to exercise it
+@pytest.mark.parametrize("work_order", [0, 2, 4, 3 | 8, 7 | 8])
 def test_work_orders_are_exact(gpu, oracle_lib, work_order):
     """Every storage order and placement (the reference's order, XCD placement,
     LDS-staged pressure passes, all together with the Morton layout) leaves the
@@ -293,7 +295,8 @@ def test_work_orders_are_exact(gpu, oracle_lib, work_order):
     o = oracle_lib.OracleFluid()
     g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
     so = run(o, scene, 2, max_iterations=20)
-    sg = run(g, scene, 2, max_iterations=20, work_order=work_order)
+    # the LDS-tiled pressure passes (bit 2) run with 256-thread workgroups only
+    sg = run(g, scene, 2, max_iterations=20, work_order=work_order, block_size=256 if work_order & 4 else 0)
     assert [s["iterations"] for s in so] == [s["iterations"] for s in sg]
     ro, rg = snapshot(o), snapshot(g)
     for name in ro:
@@ -311,7 +314,7 @@ def test_layout_transitions_bit_exact(gpu, oracle_lib):
     o = oracle_lib.OracleFluid()
     g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
     run(o, scene, 2, max_iterations=20)
-    run(g, scene, 2, max_iterations=20)
+    run(g, scene, 2, max_iterations=20, work_order=3 | 8)  # the layout at this size
 
     def step_both(k):
         for _ in range(k):
@@ -327,7 +330,7 @@ def test_layout_transitions_bit_exact(gpu, oracle_lib):
     g.set_solver_options(max_iterations=20, work_order=0)
     step_both(1)
     same("layout off")
-    g.set_solver_options(max_iterations=20, work_order=3)
+    g.set_solver_options(max_iterations=20, work_order=3 | 8)
     step_both(2)
     same("layout on again")
     v = o.download(Field.FLUID_VELOCITIES) * np.float32(0.5)
