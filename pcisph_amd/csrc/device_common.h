@@ -105,19 +105,39 @@ __device__ __forceinline__ void for_each_bucket_row(int3 c, const HashGrid& g, F
 // range of the bucket that cell lo + (rx, ry, rz) hashes to, so a neighbourhood's
 // lookups read neighbouring words instead of hash-scattered ones.  Both give the
 // same range for a cell; cells outside the box use the bucket table.
+//
+// A grid entry may carry kFarBucket in its start word: every particle of that bucket
+// lies in a cell at least kFarCells cells from the entry's cell along some axis (hash
+// collisions from elsewhere in the scene), hence at least 1.99 cell sizes from every
+// position whose cell is the entry's cell or one of its 26 neighbours -- the positions
+// that look the entry up (the 0.01 covers the rounding of cell coordinates below
+// 2^15, the only ones flagged).  A walk whose evaluation points stay within d of the
+// walking particle's position, and whose terms vanish beyond radius rho, adds exactly
+// zero for such a bucket when d + rho <= kFarReach cell sizes: it sets `prune` and
+// reads those entries as empty.  Candidate counts still include them.
+//
+// kQuietCell in an entry's end word marks a cell all 27 of whose neighbourhood
+// entries are empty or far: a pruning walk from it has nothing to visit.
+constexpr uint32_t kFarBucket = 0x80000000u;
+constexpr uint32_t kQuietCell = 0x80000000u;
+constexpr int kFarCells = 4;
+constexpr float kFarReach = 1.98f;
+
 struct CellTable {
     const uint2* cells;
     const uint2* grid;
     HashGrid g;
     uint32_t lo[3];   // box origin (cell coordinates as 32-bit words)
     uint32_t dim[3];  // box size in cells; dim[0] == 0: no box
+    bool prune;       // grid entries flagged kFarBucket read as empty
 };
 
 // The bucket ranges of the 27 cells around c, one (dz, dy) row of three x-adjacent
 // cells at a time, in the reference's order (grid_utils.cl:65-79): fn(r0, r1, r2).
 // A row inside the box reads three consecutive grid entries; any other row hashes.
+// `full`, when given, accumulates the ranges' lengths before pruning.
 template <typename F>
-__device__ __forceinline__ void for_each_range_row(const CellTable& T, int3 c, F&& fn) {
+__device__ __forceinline__ void for_each_range_row(const CellTable& T, int3 c, F&& fn, uint32_t* full = nullptr) {
     const uint32_t x0 = (uint32_t)c.x - 1u;
     const uint32_t rx = x0 - T.lo[0];
     const bool xin = T.dim[0] >= 3u && rx <= T.dim[0] - 3u;
@@ -136,16 +156,32 @@ __device__ __forceinline__ void for_each_range_row(const CellTable& T, int3 c, F
         if (xin && ry < T.dim[1] && rz < T.dim[2]) {
             const uint2* q = T.grid + (rx + T.dim[0] * (ry + T.dim[1] * rz));
 #pragma unroll
-            for (int d = 0; d < 3; ++d) r[d] = q[d];
+            for (int d = 0; d < 3; ++d) {
+                r[d] = q[d];
+                const uint32_t far = r[d].x & kFarBucket;
+                r[d].x ^= far;
+                r[d].y &= ~kQuietCell;
+                if (full) *full += r[d].y - r[d].x;
+                if (far && T.prune) r[d].y = r[d].x;
+            }
         } else {
             const uint32_t hyz = (y * 73856093u) ^ (z * 83492791u);
             const uint32_t lyz = 4u * (y & 63u) + 16u * (z & 63u);
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d) {
                 r[d] = T.cells[bucket_mod((((hx[d] ^ hyz) << 6) + lx[d] + lyz) & 0x7FFFFFFFu, T.g)];
+                if (full) *full += r[d].y - r[d].x;
+            }
         }
         fn(r[0], r[1], r[2]);
     }
+}
+
+// Whether cell c is inside T's box and marked kQuietCell.
+__device__ __forceinline__ bool quiet_cell(const CellTable& T, int3 c) {
+    const uint32_t rx = (uint32_t)c.x - T.lo[0], ry = (uint32_t)c.y - T.lo[1], rz = (uint32_t)c.z - T.lo[2];
+    if (rx >= T.dim[0] || ry >= T.dim[1] || rz >= T.dim[2]) return false;
+    return (T.grid[rx + T.dim[0] * (ry + T.dim[1] * rz)].y & kQuietCell) != 0;
 }
 
 template <typename F>

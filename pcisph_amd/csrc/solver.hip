@@ -1063,7 +1063,7 @@ class Engine final : public EngineBase {
             });
         }
         if (A_.box_cells && A_.nb && (bgrid_stale_ || sort_boundary)) {
-            DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.bcells, A_.bgrid);
+            DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.bcells, A_.bgrid, A_.bpos);
             HIP_CHECK(hipGetLastError());
             bgrid_stale_ = false;
         }

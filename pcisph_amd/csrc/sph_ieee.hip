@@ -225,10 +225,13 @@ void DataLaunch<R>::translate_scenes(const Launch& L, const Batch<R>& B, const f
 
 template <typename R>
 void DataLaunch<R>::fill_box(const Launch& L, const SphParams<R>* scenes, uint32_t scene_count, uint32_t max_cells,
-                             const uint2* cells, uint2* grid) {
+                             const uint2* cells, uint2* grid, const V4<R>* far_pos) {
     if (!scene_count || !max_cells) return;
     const uint32_t blocks = std::min(grid_for(max_cells, 256), 1024u);
-    hipLaunchKernelGGL((k_fill_box<R>), dim3(blocks, scene_count), dim3(256), 0, L.stream, scenes, cells, grid);
+    hipLaunchKernelGGL((k_fill_box<R>), dim3(blocks, scene_count), dim3(256), 0, L.stream, scenes, cells, grid,
+                       far_pos);
+    if (far_pos)
+        hipLaunchKernelGGL((k_mark_quiet<R>), dim3(blocks, scene_count), dim3(256), 0, L.stream, scenes, grid);
 }
 
 template <typename R>

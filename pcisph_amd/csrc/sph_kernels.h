@@ -176,6 +176,7 @@ __device__ __forceinline__ CellTable cell_table(const SphParams<R>& P, const uin
         T.lo[a] = P.box_lo[a];
         T.dim[a] = grid ? P.box_dim[a] : 0u;
     }
+    T.prune = false;
     return T;
 }
 
@@ -464,16 +465,57 @@ __global__ void k_reorder(uint32_t n, const uint32_t* __restrict__ keys, const u
 }
 
 // The dense range grid of scene blockIdx.y's cell box (CellTable): each box cell's
-// bucket range, copied from the bucket table after a sort.
+// bucket range, copied from the bucket table after a sort.  With `pos` (the positions
+// the table's ranges index), an entry whose bucket holds only particles at least
+// kFarCells cells away along some axis gets kFarBucket; cells with a coordinate of
+// 2^15 or more, buckets of more than kFarScan particles and buckets with a NaN
+// position (whose terms are NaN at any distance) are never flagged.
+constexpr uint32_t kFarScan = 64;
+
 template <typename R>
 __global__ void k_fill_box(const SphParams<R>* __restrict__ scenes, const uint2* __restrict__ cells,
-                           uint2* __restrict__ grid) {
+                           uint2* __restrict__ grid, const V4<R>* __restrict__ pos) {
     const SphParams<R>& P = scenes[blockIdx.y];
     const uint32_t dx = P.box_dim[0], dy = P.box_dim[1], total = dx * dy * P.box_dim[2];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
         const uint32_t rx = q % dx, rest = q / dx, ry = rest % dy, rz = rest / dy;
-        const uint32_t k = hash_key(P.box_lo[0] + rx, P.box_lo[1] + ry, P.box_lo[2] + rz, P.grid);
-        grid[P.box_base + q] = cells[P.bucket_base + k];
+        const uint32_t cx = P.box_lo[0] + rx, cy = P.box_lo[1] + ry, cz = P.box_lo[2] + rz;
+        uint2 r = cells[P.bucket_base + hash_key(cx, cy, cz, P.grid)];
+        const int32_t n[3] = {(int32_t)cx, (int32_t)cy, (int32_t)cz};
+        bool far = pos && r.y > r.x && r.y - r.x <= kFarScan;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) far = far && n[a] > -32768 && n[a] < 32768;
+        for (uint32_t j = r.x; far && j < r.y; ++j) {
+            const V4<R> o = pos[j];
+            const int3 m = cell_of4<R>(o, P.cell);
+            const int64_t d[3] = {(int64_t)m.x - n[0], (int64_t)m.y - n[1], (int64_t)m.z - n[2]};
+            bool away = false;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) away = away || d[a] >= kFarCells || d[a] <= -kFarCells;
+            far = away && o.x == o.x && o.y == o.y && o.z == o.z;
+        }
+        if (far) r.x |= kFarBucket;
+        grid[P.box_base + q] = r;
+    }
+}
+
+// kQuietCell on every cell of scene blockIdx.y's box whose 27 neighbours lie in the
+// box and whose entries are all empty or kFarBucket (after k_fill_box with positions).
+// Only end-word top bits change, which no reader here looks at.
+template <typename R>
+__global__ void k_mark_quiet(const SphParams<R>* __restrict__ scenes, uint2* __restrict__ grid) {
+    const SphParams<R>& P = scenes[blockIdx.y];
+    const uint32_t dx = P.box_dim[0], dy = P.box_dim[1], dz = P.box_dim[2], total = dx * dy * dz;
+    uint2* G = grid + P.box_base;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        const uint32_t rx = q % dx, rest = q / dx, ry = rest % dy, rz = rest / dy;
+        if (rx < 1 || ry < 1 || rz < 1 || rx + 1 >= dx || ry + 1 >= dy || rz + 1 >= dz) continue;
+        bool quiet = true;
+        for (int o = 0; o < 27 && quiet; ++o) {
+            const uint2 e = G[(rx + o % 3 - 1) + dx * ((ry + (o / 3) % 3 - 1) + dy * (rz + o / 9 - 1))];
+            quiet = (e.x & kFarBucket) || (e.x & ~kFarBucket) == (e.y & ~kQuietCell);
+        }
+        if (quiet) G[q].y |= kQuietCell;
     }
 }
 
@@ -613,9 +655,20 @@ __device__ __forceinline__ void gather_row(uint2 r0, uint2 r1, uint2 r2, Load&& 
 
 // The 27 buckets around cell c: a row's three bucket ranges are loaded before any of
 // them is walked, then gathered as one sequence.
+// `full`, when given, accumulates the candidate count before far buckets are pruned.
 template <typename R, int U = kGatherBatch, typename Load, typename Use>
-__device__ __forceinline__ void walk_buckets(const CellTable& T, int3 c, Load&& load, Use&& use) {
-    for_each_range_row(T, c, [&](uint2 r0, uint2 r1, uint2 r2) { gather_row<U>(r0, r1, r2, load, use); });
+__device__ __forceinline__ void walk_buckets(const CellTable& T, int3 c, Load&& load, Use&& use,
+                                             uint32_t* full = nullptr) {
+    for_each_range_row(
+        T, c, [&](uint2 r0, uint2 r1, uint2 r2) { gather_row<U>(r0, r1, r2, load, use); }, full);
+}
+
+// Whether a walk may read far buckets as empty (CellTable): its evaluation points lie
+// within d of the walking particle's position and its terms vanish beyond rho.  False
+// for a NaN d.
+template <typename R>
+__device__ __forceinline__ bool far_prunable(float d, float rho, R cell) {
+    return d + rho <= kFarReach * (float)cell;
 }
 
 // Particle i's list of `len` entries (chunks of four, slot-major), U loads in flight
@@ -695,16 +748,21 @@ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __rest
             R rho = 0;
             uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
             ListWriter wb{NL.b, B.n_total, t, NL.cap_b, 0, {}}, wf{NL.f, B.n_total, t, NL.cap_f, 0, {}};
-            if (P.nb)
-                walk_buckets<R>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
-                                [&](const V4<R>& o, uint32_t j) {
-                                    R dx, dy, dz;
-                                    const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
-                                    rho += poly6<R>(r2, P.h2);
-                                    ++cb;
-                                    nb += !(r2 > P.h2);
-                                    if (lists && r2 <= lr2) wb.push(j);
-                                });
+            if (P.nb) {
+                // terms vanish beyond h; list entries beyond the list radius
+                CellTable BT = cell_table(P, bcells, B.bgrid);
+                BT.prune = far_prunable<R>(0.f, sqrtf(fmaxf((float)P.h2, lists ? (float)lr2 : 0.f)), P.cell);
+                walk_buckets<R>(
+                    BT, c, [&](uint32_t j) { return bpos[j]; },
+                    [&](const V4<R>& o, uint32_t j) {
+                        R dx, dy, dz;
+                        const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+                        rho += poly6<R>(r2, P.h2);
+                        nb += !(r2 > P.h2);
+                        if (lists && r2 <= lr2) wb.push(j);
+                    },
+                    &cb);
+            }
             walk_buckets<R>(FT, c, [&](uint32_t j) { return pos[j]; },
                             [&](const V4<R>& o, uint32_t j) {
                                 R dx, dy, dz;
@@ -952,12 +1010,15 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
             R pd = 0;
             if (P.nb) {
                 const uint32_t len = lists ? NL.bcount[t] : kListOverflow;
-                walk<R>(NL.b, len, B.n_total, t, cell_table(P, bcells, B.bgrid), c,
-                        [&](uint32_t j) { return bpos[j]; },
-                        [&](const V4<R>& o, uint32_t) {
-                            R dx, dy, dz;
-                            pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-                        });
+                CellTable BT = cell_table(P, bcells, B.bgrid);
+                BT.prune = far_prunable<R>(displacement<R>(pr, self), (float)P.h, P.cell);
+                if (len != kListOverflow || !(BT.prune && quiet_cell(BT, c))) {
+                    walk<R>(NL.b, len, B.n_total, t, BT, c, [&](uint32_t j) { return bpos[j]; },
+                            [&](const V4<R>& o, uint32_t) {
+                                R dx, dy, dz;
+                                pd += poly6<R>(dist2<R>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                            });
+                }
             }
             const uint32_t len = lists ? NL.fcount[t] : kListOverflow;
             walk<R>(NL.f, len, B.n_total, t, FT, c, [&](uint32_t j) { return xp[j]; },
@@ -1382,13 +1443,19 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
             const V4<R> pw = xp[i], pz = D.xp0[i];
             const int3 c = cell_of4<R>(self, P.cell);
             R sw = 0, sz = 0;
-            if (P.nb)
-                walk_buckets<R>(cell_table(P, bcells, B.bgrid), c, [&](uint32_t j) { return bpos[j]; },
-                                [&](const V4<R>& o, uint32_t) {
-                                    R dx, dy, dz;
-                                    sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-                                    sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-                                });
+            if (P.nb) {
+                CellTable BT = cell_table(P, bcells, B.bgrid);
+                BT.prune = far_prunable<R>(displacement<R>(pw, self), (float)P.h, P.cell) &&
+                           far_prunable<R>(displacement<R>(pz, self), (float)P.h, P.cell);
+                if (!(BT.prune && quiet_cell(BT, c))) {
+                    walk_buckets<R>(BT, c, [&](uint32_t j) { return bpos[j]; },
+                                    [&](const V4<R>& o, uint32_t) {
+                                        R dx, dy, dz;
+                                        sw += poly6<R>(dist2<R>(pw.x, pw.y, pw.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                                        sz += poly6<R>(dist2<R>(pz.x, pz.y, pz.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+                                    });
+                }
+            }
             struct Pair {
                 V4<R> w, z;
             };

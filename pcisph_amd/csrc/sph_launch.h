@@ -127,9 +127,11 @@ struct DataLaunch {
     // every scene's boundary positions += shift[3 s .. 3 s + 2] (device array)
     static void translate_scenes(const Launch& L, const Batch<R>& B, const float* shift, V4<R>* bpos);
     // the range grids of all `scene_count` scenes' boxes (the largest has `max_cells`
-    // cells) from a bucket table
+    // cells) from a bucket table; with `far_pos`, the positions the table's ranges
+    // index, entries of far buckets are flagged (kFarBucket) and cells whose whole
+    // neighbourhood is empty or far are marked (kQuietCell)
     static void fill_box(const Launch& L, const SphParams<R>* scenes, uint32_t scene_count, uint32_t max_cells,
-                         const uint2* cells, uint2* grid);
+                         const uint2* cells, uint2* grid, const V4<R>* far_pos = nullptr);
     // cell extents of every scene's fluid and boundary particles into ext[12 S]
     // (k_extents; ext prepared by the caller)
     static void extents(const Launch& L, const Batch<R>& B, const V4<R>* pos, const V4<R>* bpos, int* ext);
