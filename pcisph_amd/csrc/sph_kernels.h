@@ -593,7 +593,9 @@ __device__ __forceinline__ void block_max_to(unsigned long long (&m)[Q], unsigne
     if (threadIdx.x < Q) {
         unsigned long long v = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = lds[threadIdx.x][w] > v ? lds[threadIdx.x][w] : v;
-        if (v) atomicMax(dst[threadIdx.x], v);
+        // a device-coherent read sees the step's reset (ordered before this launch by
+        // the stream) or a later, larger value: nothing at or below it is missing
+        if (v && v > *(volatile unsigned long long*)dst[threadIdx.x]) atomicMax(dst[threadIdx.x], v);
     }
 }
 
