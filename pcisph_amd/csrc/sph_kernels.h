@@ -863,12 +863,14 @@ __device__ __forceinline__ void integrate(const SphParams<R>& P, const V4<R>& x,
 // force_initialization (pcisph.cl:121-173) fused with the first prediction of the
 // correction loop (pcisph.cl:175-193 with F_p = 0, Fluid.cpp:253-260); fluid ranges.
 // With features requested it also gathers the warm-start network's inputs; the
-// velocity's w component carries the previous step's converged pressure.
+// velocity's w component carries the previous step's converged pressure.  The
+// pressure-force reset to zero is not stored: that array is read only after the
+// step, and the loop's last pressure-force pass always writes it.
 template <typename R, bool FAST>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
                              const V4<R>* __restrict__ vel, const V4<R>* __restrict__ nrm,
-                             V4<R>* __restrict__ fo, R* __restrict__ p, V4<R>* __restrict__ fp,
-                             V4<R>* __restrict__ xp, FeatureArgs F, NeighborLists NL) {
+                             V4<R>* __restrict__ fo, R* __restrict__ p, V4<R>* __restrict__ xp,
+                             FeatureArgs F, NeighborLists NL) {
     using O = Ops<R, FAST>;
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
@@ -937,9 +939,6 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, con
         f.w = 0;
         fo[i] = f;
         p[i] = 0;
-        V4<R> z;
-        z.x = z.y = z.z = z.w = 0;
-        fp[i] = z;
         V4<R> xo, vo;
         integrate<R, FAST>(P, self, sv, ox + R(0), oy + R(0), oz + R(0), xo, vo);
         xp[i] = xo;
@@ -1375,7 +1374,6 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
         f.y = fy * P.pf_scale;
         f.z = fz * P.pf_scale;
         f.w = 0;
-        fp[i] = f;
         const V4<R> o = fo[i];
         const V4<R> v = vel[i];
         V4<R> xo, vo;
@@ -1385,6 +1383,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
             xp[i] = xo;
             moved = displacement<R>(xo, self);
         } else {
+            fp[i] = f;  // the step's pressure force: only the last pass's is ever read
             pos_out[i] = xo;
             vo.w = p[i];  // the converged pressure rides along to the next step
             vel_out[i] = vo;

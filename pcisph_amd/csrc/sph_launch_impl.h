@@ -32,7 +32,7 @@ void SphLaunch<R, FAST>::force_init(const Launch& L, const DeviceArrays<R>& A, c
     const Batch<R>& B = A.batch;
     if (!B.fluid_blocks) return;
     hipLaunchKernelGGL((k_force_init<R, FAST>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B, A.cells, A.xr,
-                       A.vel, A.nrm, A.fo, A.p, A.fp, A.xp, F, A.NL);
+                       A.vel, A.nrm, A.fo, A.p, A.xp, F, A.NL);
 }
 
 template <typename R, bool FAST>

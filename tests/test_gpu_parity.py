@@ -237,6 +237,36 @@ def test_list_overflow_falls_back_to_buckets(gpu, oracle_lib):
         np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_far_collision_pruning_bit_exact(gpu, oracle_lib, fp64):
+    """Boundary buckets that hold only far hash collisions read as empty for walks
+    that cannot reach them (512 fluid particles, 256 buckets, 6558 wall particles:
+    most boundary lookups hit such buckets).  Velocities up to 120 m/s put the
+    predicted positions anywhere from 0 to 2.5 h from x, on both sides of the reach
+    limit, so both the pruned and the full walks run in the same pressure passes;
+    every field stays the oracle's bit for bit."""
+    scene = small_scenes()[0]
+    print(f"case far collision pruning fp64={fp64}", flush=True)
+    o = oracle_lib.OracleFluid(fp64=fp64)
+    g = gpu.Fluid(precision=gpu.PRECISION_FP64 if fp64 else gpu.PRECISION_IEEE)
+    o.load_scene(scene, max_iterations=20)
+    g.load_scene(scene, max_iterations=20)
+    rng = np.random.default_rng(7)
+    v = rng.uniform(-120.0, 120.0, 3 * scene.fluid_count).astype(np.float32)
+    o.upload(Field.FLUID_VELOCITIES, v)
+    g.upload(Field.FLUID_VELOCITIES, v)
+    for _ in range(2):
+        a, b = o.step(), g.step()
+        assert a["iterations"] == b["iterations"]
+        assert a["max_density_variation"] == b["max_density_variation"]
+        assert a["boundary_candidates"] == b["boundary_candidates"]
+    for f in FLOAT_FIELDS:
+        if fp64:
+            np.testing.assert_array_equal(g.download_f64(f), o.download_f64(f), err_msg=Field(f).name)
+        else:
+            np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
+
+
 def test_list_radius_adapts_bit_exact(gpu, oracle_lib):
     """Particles thrown at 10 m/s move further per step than a 0.05 h skin allows, so
     the pressure passes walk their buckets; the scene then builds near-h lists with no
