@@ -209,6 +209,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
+    ap.add_argument("--profile-every", type=int, default=8,
+                    help="time every n-th step of the timed region kernel by kernel between HIP events")
     ap.add_argument("--forms", default="plain,warm_start,batch_64,batch_256,batch_256_warm,sloshing_plain,"
                                        "sloshing_warm,gain,gain_sloshing",
                     help="comma list of plain (reference loop), previous_pressure (each step starts from "
@@ -271,7 +273,8 @@ def main() -> int:
                 fluid.set_warm_start(True)
             print(f"bench: {form} {len(batch)} scenes N={total} steps={args.batch_steps}", file=sys.stderr,
                   flush=True)
-            r = measure(fluid, args.batch_steps, args.warmup, total, not args.no_profile, batch=batch)
+            r = measure(fluid, args.batch_steps, args.warmup, total, not args.no_profile, batch=batch,
+                        profile_every=args.profile_every)
             # scenes whose loop has ended skip the remaining iterations, so the per-particle
             # byte model of a launch does not hold for a batch: stage times only
             r.pop("roofline", None)
@@ -294,7 +297,8 @@ def main() -> int:
             print(f"bench: {form} {sc.name} N={n} Nb={sc.boundary_count} warmup={args.warmup} "
                   f"steps={args.steps}", file=sys.stderr, flush=True)
             r = measure(fluid, args.steps, args.warmup, n, not args.no_profile,
-                        traffic=form == "plain" and args.workload == "dam_break_1m", scene=sc)
+                        traffic=form == "plain" and args.workload == "dam_break_1m", scene=sc,
+                        profile_every=args.profile_every)
         fluid.close()
         results[form] = r
         print(f"bench: {form} {json.dumps({k: v for k, v in r.items() if k != 'stage_times'})}",
@@ -321,6 +325,7 @@ def main() -> int:
                    "fluid_particles": n, "boundary_particles": scene.boundary_count,
                    "precision": args.precision, "density_error_bound": 0.01,
                    "iterations_min_max": [3, 50],
+                   "stage_timing": None if args.no_profile else f"HIP events, every {args.profile_every}th step",
                    **{k: v for k, v in main_r.items() if k not in ("value", "ms_per_step", "roofline")}},
         "roofline": main_r.get("roofline"), "cpu_baseline": cpu,
         "forms": {k: {q: v for q, v in r.items() if q != "stage_times"} for k, r in results.items()},
