@@ -381,6 +381,7 @@ class Engine final : public EngineBase {
         dfree(D_.bxf0);
         dfree(scene_table_);
         dfree(block_table_);
+        dfree(running_table_);
         dfree(states_);
         dfree(shift_dev_);
         if (shift_host_) (void)hipHostFree(shift_host_);
@@ -807,6 +808,8 @@ class Engine final : public EngineBase {
                 t.push_back({s, b, std::min(b + block, b_off_[s + 1]), kRangeBoundary});
         HIP_CHECK(hipStreamSynchronize(stream_));  // kernels enqueued ahead may still read the old table
         dalloc(block_table_, std::max<size_t>(t.size(), 1), stream_);
+        dalloc(running_table_, std::max<size_t>(t.size(), 1), stream_);
+        blocks_host_ = t;
         if (!t.empty())
             HIP_CHECK(hipMemcpyAsync(block_table_, t.data(), sizeof(BlockRange) * t.size(), hipMemcpyHostToDevice,
                                      stream_));
@@ -1122,13 +1125,39 @@ class Engine final : public EngineBase {
                 });
             }
         };
+        // Batches: once at least half of the workgroups belong to scenes whose loops
+        // have ended (they return at once, but every launch still dispatches them), the
+        // iterations enqueued from then on are launched one by one over a block table
+        // of the running scenes alone (AR), re-cut when it halves again.
+        const DeviceArrays<R>* AI = &A_;
+        DeviceArrays<R> AR = A_;
+        size_t table_blocks = blocks_host_.size();
+        auto keep_running = [&]() {
+            std::vector<BlockRange> t;
+            uint32_t nf = 0;
+            for (const BlockRange& b : blocks_host_)
+                if (results_[b.scene].running) {
+                    t.push_back(b);
+                    nf += (b.flags & kRangeBoundary) ? 0u : 1u;
+                }
+            if (t.empty() || t.size() * 2 > table_blocks) return;
+            // stream-ordered: iterations enqueued before read the previous table first
+            HIP_CHECK(hipMemcpyAsync(running_table_, t.data(), sizeof(BlockRange) * t.size(),
+                                     hipMemcpyHostToDevice, stream_));
+            AR.batch.blocks = running_table_;
+            AR.batch.fluid_blocks = nf;
+            AR.batch.boundary_blocks = (uint32_t)t.size() - nf;
+            AI = &AR;
+            table_blocks = t.size();
+        };
         auto iteration = [&](uint32_t it) {
+            const DeviceArrays<R>& A = *AI;
             prof_.timed(PCISPH_STAGE_PRESSURE, stream_, (int)it, [&] {
-                if (warm && it == 0) ops.pressure_dual(L, A_, D_);
-                else ops.pressure(L, A_, it);
+                if (warm && it == 0) ops.pressure_dual(L, A, D_);
+                else ops.pressure(L, A, it);
             });
             prof_.timed(PCISPH_STAGE_PRESSURE_FORCE, stream_, (int)it, [&] {
-                ops.pressure_force(L, A_, it, dev_host_, dev_results_, ctl_, 0, nullptr);
+                ops.pressure_force(L, A, it, dev_host_, dev_results_, ctl_, 0, nullptr);
             });
         };
         if (chunk) {
@@ -1152,7 +1181,7 @@ class Engine final : public EngineBase {
         const uint32_t step_iters = chunk ? chunk : 1u;
         uint32_t enq = 0;
         auto enqueue = [&]() {
-            if (chunk) {
+            if (chunk && AI == &A_) {
                 const uint32_t c = enq / chunk;
                 if (loop_graphs_.size() <= c) loop_graphs_.resize(c + 1, nullptr);
                 if (!loop_graphs_[c])
@@ -1160,10 +1189,11 @@ class Engine final : public EngineBase {
                         for (uint32_t it = enq; it < std::min(enq + chunk, maxit); ++it) iteration(it);
                     });
                 HIP_CHECK(hipGraphLaunch(loop_graphs_[c], stream_));
+                enq = std::min(enq + step_iters, maxit);
             } else {
                 iteration(enq);
+                ++enq;
             }
-            enq = std::min(enq + step_iters, maxit);
         };
         while (enq < maxit && enq <= first + lookahead) enqueue();
         HIP_CHECK(hipGetLastError());
@@ -1172,6 +1202,7 @@ class Engine final : public EngineBase {
             wait_for_iteration(k, tag);
             if (host_->running[k] == 0) break;
             if (k + 1 >= maxit) throw Error(PCISPH_ERR_INTERNAL, "scenes still running after the last iteration");
+            if (S > 1) keep_running();
             ++k;
             if (enq < maxit && enq <= k + lookahead) enqueue();
         }
@@ -1361,6 +1392,8 @@ class Engine final : public EngineBase {
     std::vector<SphParams<R>> scene_host_;
     SphParams<R>* scene_table_ = nullptr;
     BlockRange* block_table_ = nullptr;
+    BlockRange* running_table_ = nullptr;  // the ranges of the scenes still running (batches)
+    std::vector<BlockRange> blocks_host_;
     uint32_t blocks_for_ = 0, fluid_blocks_ = 0, boundary_blocks_ = 0;
     LoopState* states_ = nullptr;
     uint32_t states_n_ = 0;

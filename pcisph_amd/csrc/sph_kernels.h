@@ -135,7 +135,7 @@ struct SceneResult {
     unsigned int iterations;
     unsigned int warm_rejected;
     unsigned int list_waves;
-    unsigned int pad;
+    unsigned int running;  // 1: the scene runs another iteration after this one
 };
 
 // Inputs of the warm-start network, per fluid particle (8 floats, row-major):
@@ -1294,10 +1294,10 @@ __device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* 
         r.iterations = iter + 1;
         r.warm_rejected = st->warm_rejected;
         r.list_waves = st->list_waves;
-        r.pad = 0;
+        r.running =
+            iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations, P.rho0, P.threshold) ? 0u : 1u;
         res[s] = r;
-        if (!iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations, P.rho0, P.threshold))
-            ++running;
+        running += r.running;
     }
     __threadfence_system();
 #pragma unroll

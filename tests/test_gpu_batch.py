@@ -90,6 +90,32 @@ def test_batch_equals_scenes_alone_with_layout(gpu):
         compare_scene(g, k, alone, sb, sa, FLOAT_FIELDS + INT_FIELDS)
 
 
+@pytest.mark.parametrize("graph_iterations", [0, 4])
+def test_batch_tail_on_running_scenes_only(gpu, graph_iterations):
+    """Four scenes whose loops end at the minimum iteration and one that runs to the
+    cap: from the first check on, more than half of the workgroups belong to ended
+    loops, so the later iterations run over a block table of the running scene alone
+    (launched one by one, after the graphs already enqueued).  Every scene still leaves
+    the bits it leaves alone, and the long scene's loop runs to the cap."""
+    slow = scenes.small_dam_break(seed=99)
+    slow.threshold = 1e-6
+    fast = []
+    for k in range(4):
+        sc = scenes.small_dam_break(seed=100 + k) if k % 2 == 0 else scenes.dam_break(
+            (10, 10, 10), 0.02, 0.001, (0.4, 0.4, 0.3), seed=200 + k, name=f"dam_break_1000_{k}", steps=5)
+        sc.threshold = 0.5
+        fast.append(sc)
+    batch = [fast[0], slow, fast[1], fast[2], fast[3]]
+    print(f"case batch tail graph_iterations={graph_iterations}", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    sb = run_batch(g, batch, 2, max_iterations=12, graph_iterations=graph_iterations)
+    assert [s["iterations"] for s in sb[0]] == [3, 12, 3, 3, 3], [s["iterations"] for s in sb[0]]
+    for k, sc in enumerate(batch):
+        alone = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+        sa = run(alone, sc, 2, max_iterations=12, graph_iterations=graph_iterations)
+        compare_scene(g, k, alone, sb, sa, FLOAT_FIELDS + INT_FIELDS)
+
+
 def test_batch_without_lists_and_with_empty_scene(gpu):
     a, b = small_scenes()
     empty = scenes.Scene(name="empty", fluid=np.zeros(0, np.float32), boundary=np.zeros(0, np.float32),
