@@ -382,6 +382,7 @@ class Engine final : public EngineBase {
         dfree(scene_table_);
         dfree(block_table_);
         dfree(running_table_);
+        dfree(flags_dev_);
         dfree(states_);
         dfree(shift_dev_);
         if (shift_host_) (void)hipHostFree(shift_host_);
@@ -748,11 +749,21 @@ class Engine final : public EngineBase {
     }
     void profile_read(pcisph_stage_times* out) override { prof_.read(stream_, out); }
 
+    // Every scene's non-finite flag: gathered into one word array on the device and
+    // copied at once (a strided copy of one word per scene took milliseconds for
+    // hundreds of scenes).
     std::vector<uint32_t> nonfinite() override {
         std::vector<uint32_t> v(n_.size(), 0);
-        if (!v.empty())
-            HIP_CHECK(hipMemcpy2DAsync(v.data(), sizeof(uint32_t), &states_->nonfinite, sizeof(LoopState),
-                                       sizeof(uint32_t), v.size(), hipMemcpyDeviceToHost, stream_));
+        if (!v.empty()) {
+            if (flags_cap_ < v.size()) {
+                dalloc(flags_dev_, v.size(), stream_);
+                flags_cap_ = v.size();
+            }
+            nonfinite_flags(Launch{stream_, 256}, states_, (uint32_t)v.size(), flags_dev_);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(v.data(), flags_dev_, sizeof(uint32_t) * v.size(), hipMemcpyDeviceToHost,
+                                     stream_));
+        }
         HIP_CHECK(hipStreamSynchronize(stream_));
         return v;
     }
@@ -1393,6 +1404,8 @@ class Engine final : public EngineBase {
     SphParams<R>* scene_table_ = nullptr;
     BlockRange* block_table_ = nullptr;
     BlockRange* running_table_ = nullptr;  // the ranges of the scenes still running (batches)
+    uint32_t* flags_dev_ = nullptr;  // per-scene non-finite flags, gathered
+    size_t flags_cap_ = 0;
     std::vector<BlockRange> blocks_host_;
     uint32_t blocks_for_ = 0, fluid_blocks_ = 0, boundary_blocks_ = 0;
     LoopState* states_ = nullptr;

@@ -71,6 +71,11 @@ __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
     if (i < n) p[i] = i;
 }
 
+__global__ void k_nonfinite_flags(const LoopState* __restrict__ states, uint32_t count, uint32_t* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < count) out[s] = states[s].nonfinite;
+}
+
 // Bucket ranges in canonical indices from the sorted keys (the reference's cell table).
 __global__ void k_cells_from_keys(uint32_t n, const uint32_t* __restrict__ keys, uint2* __restrict__ cells) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -204,6 +209,11 @@ void gather_words(const Launch& L, uint32_t n, uint32_t words, const uint32_t* m
                   uint32_t* dst) {
     if (!n) return;
     hipLaunchKernelGGL(k_gather_words, dim3(grid_for(n, 256)), dim3(256), 0, L.stream, src, dst, n, words, map);
+}
+
+void nonfinite_flags(const Launch& L, const LoopState* states, uint32_t count, uint32_t* out) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_nonfinite_flags, dim3(grid_for(count, 256)), dim3(256), 0, L.stream, states, count, out);
 }
 
 void fill_iota(const Launch& L, uint32_t* p, uint32_t n) {

@@ -139,6 +139,8 @@ struct DataLaunch {
 
 void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);
 void fill_iota(const Launch& L, uint32_t* p, uint32_t n);  // p[i] = i
+// out[s] = states[s].nonfinite: every scene's flag in one contiguous word array
+void nonfinite_flags(const Launch& L, const LoopState* states, uint32_t count, uint32_t* out);
 // dst[i] = item map[i] of src, `words` uint32 per item
 void gather_words(const Launch& L, uint32_t n, uint32_t words, const uint32_t* map, const uint32_t* src,
                   uint32_t* dst);
