@@ -116,6 +116,22 @@ def test_batch_tail_on_running_scenes_only(gpu, graph_iterations):
         compare_scene(g, k, alone, sb, sa, FLOAT_FIELDS + INT_FIELDS)
 
 
+def test_nonfinite_flag_per_scene(gpu):
+    """A NaN velocity in one scene of a batch makes that scene's step non-finite (a NaN
+    position takes cell 0, so every lookup stays in its tables) and flags that scene
+    alone; the flags of all scenes are read back together after the step."""
+    batch = mixed_scenes()
+    print("case nonfinite flag per scene", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_FAST)
+    g.load_scenes(batch, max_iterations=20)
+    assert [s["nonfinite"] for s in g.step_scenes()] == [0, 0, 0]
+    g.select_scene(1)
+    v = g.download(Field.FLUID_VELOCITIES)
+    v[0] = np.nan
+    g.upload(Field.FLUID_VELOCITIES, v)
+    assert [s["nonfinite"] for s in g.step_scenes()] == [0, 1, 0]
+
+
 def test_batch_without_lists_and_with_empty_scene(gpu):
     a, b = small_scenes()
     empty = scenes.Scene(name="empty", fluid=np.zeros(0, np.float32), boundary=np.zeros(0, np.float32),
