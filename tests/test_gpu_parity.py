@@ -267,6 +267,28 @@ def test_far_collision_pruning_bit_exact(gpu, oracle_lib, fp64):
             np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
 
 
+def test_far_cells_unflagged_bit_exact(gpu, oracle_lib):
+    """The small dam break moved 2000 m along x: cell coordinates near 50,000, where
+    far-collision flags are never set (the rounding of such coordinates is not covered
+    by the reach margin), so every walk reads its buckets in full; the oracle's bits
+    still hold."""
+    scene = small_scenes()[0]
+    shift = np.array([2000.0, 0.0, 0.0], np.float32)
+    scene.fluid = (scene.fluid.reshape(-1, 3) + shift).astype(np.float32).ravel()
+    scene.boundary = (scene.boundary.reshape(-1, 3) + shift).astype(np.float32).ravel()
+    print("case far cells unflagged", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 2, max_iterations=20)
+    sg = run(g, scene, 2, max_iterations=20)
+    for a, b in zip(so, sg):
+        assert a["iterations"] == b["iterations"]
+        assert a["boundary_candidates"] == b["boundary_candidates"]
+    ro, rg = snapshot(o), snapshot(g)
+    for name in ro:
+        np.testing.assert_array_equal(rg[name], ro[name], err_msg=name)
+
+
 def test_list_radius_adapts_bit_exact(gpu, oracle_lib):
     """Particles thrown at 10 m/s move further per step than a 0.05 h skin allows, so
     the pressure passes walk their buckets; the scene then builds near-h lists with no
