@@ -110,8 +110,8 @@ __device__ __forceinline__ void for_each_bucket_row(int3 c, const HashGrid& g, F
 // lies in a cell at least kFarCells cells from the entry's cell along some axis (hash
 // collisions from elsewhere in the scene), hence at least 1.99 cell sizes from every
 // position whose cell is the entry's cell or one of its 26 neighbours -- the positions
-// that look the entry up (the 0.01 covers the rounding of cell coordinates below
-// 2^15, the only ones flagged).  A walk whose evaluation points stay within d of the
+// that look the entry up (the 0.01 covers the rounding of cell coordinates of
+// magnitude below 2^15, the only ones flagged).  A walk whose evaluation points stay within d of the
 // walking particle's position, and whose terms vanish beyond radius rho, adds exactly
 // zero for such a bucket when d + rho <= kFarReach cell sizes: it sets `prune` and
 // reads those entries as empty.  Candidate counts still include them.

@@ -468,7 +468,7 @@ __global__ void k_reorder(uint32_t n, const uint32_t* __restrict__ keys, const u
 // bucket range, copied from the bucket table after a sort.  With `pos` (the positions
 // the table's ranges index), an entry whose bucket holds only particles at least
 // kFarCells cells away along some axis gets kFarBucket; cells with a coordinate of
-// 2^15 or more, buckets of more than kFarScan particles and buckets with a NaN
+// magnitude 2^15 or more, buckets of more than kFarScan particles and buckets with a NaN
 // position (whose terms are NaN at any distance) are never flagged.
 constexpr uint32_t kFarScan = 64;
 
