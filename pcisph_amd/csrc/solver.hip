@@ -1290,20 +1290,23 @@ class Engine final : public EngineBase {
     }
 
     // Neighbour-list storage (capacities: fluid 128, boundary 96 per fluid particle,
-    // fluid 96 per boundary particle); each scene's list radius and validity limit
-    // follow from its kernel radius (scene table).
+    // fluid 96 per boundary particle, in chunks of four 24-bit entries = three words);
+    // each scene's list radius and validity limit follow from its kernel radius (scene
+    // table).  Batches of 2^24 or more fluid or boundary particles step without lists
+    // (entries would not fit; results do not depend on lists).
     void setup_lists(float skin) {
         NeighborLists& NL = A_.NL;
         list_skin_ = skin > 0.f ? skin : 0.f;
-        if (!(skin > 0.f)) {
+        if (!(skin > 0.f) || A_.n >= (1u << 24) || A_.nb >= (1u << 24)) {
             NL.on = 0;
             return;
         }
         constexpr uint32_t kCapF = 128, kCapB = 96, kCapBF = 96;
+        auto words = [](uint32_t cap, uint32_t n) { return (size_t)cap / 4 * 3 * std::max(n, 1u); };
         if (list_n_ != A_.n || list_nb_ != A_.nb || !NL.f) {
-            dalloc(NL.f, (size_t)kCapF * std::max(A_.n, 1u), stream_);
-            dalloc(NL.b, (size_t)kCapB * std::max(A_.n, 1u), stream_);
-            dalloc(NL.bf, (size_t)kCapBF * std::max(A_.nb, 1u), stream_);
+            dalloc(NL.f, words(kCapF, A_.n), stream_);
+            dalloc(NL.b, words(kCapB, A_.n), stream_);
+            dalloc(NL.bf, words(kCapBF, A_.nb), stream_);
             dalloc(NL.fcount, std::max(A_.n, 1u), stream_);
             dalloc(NL.bcount, std::max(A_.n, 1u), stream_);
             dalloc(NL.bfcount, std::max(A_.nb, 1u), stream_);

@@ -308,9 +308,10 @@ __device__ __forceinline__ void for_each_candidate(const uint2* __restrict__ cel
     });
 }
 
-// Per-step neighbour lists in chunks of four entries, slot-major: entries 4c..4c+3 of
-// particle i are the uint4 at [c * stride + i], so the lanes of a wavefront read
-// consecutive 16-byte chunks and a lane writes its list 16 bytes at a time: for
+// Per-step neighbour lists in chunks of four 24-bit entries, slot-major: entries
+// 4c..4c+3 of particle i are packed in the uint3 at [c * stride + i] (pack_chunk), so
+// the lanes of a wavefront read consecutive 12-byte chunks and a lane writes its list
+// 12 bytes at a time (lists are kept only while every index fits in 24 bits): for
 // every fluid particle its
 // fluid and boundary candidates within the list radius, for every boundary particle
 // its fluid candidates, each in the reference's bucket order.  A particle with more
@@ -381,12 +382,24 @@ __device__ __forceinline__ float bucket_displacement(const uint32_t* __restrict_
 
 // Appends to particle i's list, four entries per store (the chunk's tail past the
 // list's length is left undefined).  `cap` is a multiple of 4.
+// Four list entries (indices below 2^24) in three words, and back.
+static_assert(sizeof(uint3) == 12 && alignof(uint3) <= 4, "list chunks are three packed words");
+__device__ __forceinline__ uint3 pack_chunk(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return make_uint3(a | (b << 24), (b >> 8) | (c << 16), (c >> 16) | (d << 8));
+}
+__device__ __forceinline__ void unpack_chunk(const uint3 w, uint32_t (&e)[4]) {
+    e[0] = w.x & 0xFFFFFFu;
+    e[1] = (w.x >> 24) | ((w.y & 0xFFFFu) << 8);
+    e[2] = (w.y >> 16) | ((w.z & 0xFFu) << 16);
+    e[3] = w.z >> 8;
+}
+
 struct ListWriter {
     uint32_t* base;
     uint32_t stride, i, cap, len;
     uint32_t e[4];
     __device__ __forceinline__ void store_chunk() {
-        reinterpret_cast<uint4*>(base)[(size_t)(len >> 2) * stride + i] = make_uint4(e[0], e[1], e[2], e[3]);
+        reinterpret_cast<uint3*>(base)[(size_t)(len >> 2) * stride + i] = pack_chunk(e[0], e[1], e[2], e[3]);
     }
     __device__ __forceinline__ void push(uint32_t j) {
         const uint32_t q = len & 3u;
@@ -680,18 +693,19 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
                                           uint32_t i, Load&& load, Use&& use) {
     static_assert(U % 4 == 0, "list batches are whole chunks");
     using T = decltype(load(0u));
-    const uint4* q = reinterpret_cast<const uint4*>(list) + i;
+    const uint3* q = reinterpret_cast<const uint3*>(list) + i;
     uint32_t k = 0;
     for (; k + U <= len; k += U) {
         uint32_t jj[U];
         T d[U];
 #pragma unroll
         for (int c = 0; c < U / 4; ++c) {
-            const uint4 e = q[(size_t)(k / 4 + c) * stride];
-            jj[4 * c] = e.x;
-            jj[4 * c + 1] = e.y;
-            jj[4 * c + 2] = e.z;
-            jj[4 * c + 3] = e.w;
+            uint32_t e[4];
+            unpack_chunk(q[(size_t)(k / 4 + c) * stride], e);
+            jj[4 * c] = e[0];
+            jj[4 * c + 1] = e[1];
+            jj[4 * c + 2] = e[2];
+            jj[4 * c + 3] = e[3];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) d[u] = load(jj[u]);
@@ -699,8 +713,8 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
         for (int u = 0; u < U; ++u) use(d[u], jj[u]);
     }
     for (; k < len; k += 4) {
-        const uint4 e = q[(size_t)(k / 4) * stride];
-        const uint32_t jj[4] = {e.x, e.y, e.z, e.w};
+        uint32_t jj[4];
+        unpack_chunk(q[(size_t)(k / 4) * stride], jj);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (k + u < len) use(load(jj[u]), jj[u]);
