@@ -104,7 +104,9 @@ typedef struct pcisph_solver_options {
                                        step (lists in under 1% of their fluid wavefronts) builds
                                        lists of radius 1.01 h (shorter lists for the passes at
                                        x) until a probe step every 32 steps finds the full
-                                       radius used again.  0 = off; default 0.2 */
+                                       radius used again.  Batches of 2^24 or more fluid or
+                                       boundary particles step without lists (entries are
+                                       stored in 24 bits).  0 = off; default 0.2 */
     uint32_t graph_iterations;      /* > 0: the step's launches are replayed from HIP graphs --
                                        one for everything before the correction loop and one
                                        per `graph_iterations` loop iterations (iterations past
