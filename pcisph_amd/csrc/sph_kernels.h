@@ -514,7 +514,8 @@ __global__ void k_fill_box(const SphParams<R>* __restrict__ scenes, const uint2*
 
 // kQuietCell on every cell of scene blockIdx.y's box whose 27 neighbours lie in the
 // box and whose entries are all empty or kFarBucket (after k_fill_box with positions).
-// Only end-word top bits change, which no reader here looks at.
+// Only the end words' top bits change, and every range reader masks them off, so cells
+// read while their neighbours are being marked still see the right ranges.
 template <typename R>
 __global__ void k_mark_quiet(const SphParams<R>* __restrict__ scenes, uint2* __restrict__ grid) {
     const SphParams<R>& P = scenes[blockIdx.y];
@@ -606,8 +607,9 @@ __device__ __forceinline__ void block_max_to(unsigned long long (&m)[Q], unsigne
     if (threadIdx.x < Q) {
         unsigned long long v = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = lds[threadIdx.x][w] > v ? lds[threadIdx.x][w] : v;
-        // a device-coherent read sees the step's reset (ordered before this launch by
-        // the stream) or a later, larger value: nothing at or below it is missing
+        // the atomic is skipped when the stored maximum is already at least v: a
+        // device-coherent read returns the step's reset value (ordered before this
+        // launch by the stream) or a later, larger one, so a skipped v never exceeds it
         if (v && v > *(volatile unsigned long long*)dst[threadIdx.x]) atomicMax(dst[threadIdx.x], v);
     }
 }
