@@ -22,12 +22,18 @@
  *                        rho/rho0 = 315*330/(64*pi*512), O(N^2) brute force
  *   update_normal     -> zero at a symmetric lattice centre, two-particle form
  *   force_init        -> isolated particle F = (0, m g, 0), zero viscosity for a
- *                        uniform velocity, antisymmetric cohesion pair forces
+ *                        uniform velocity, antisymmetric cohesion pair forces;
+ *                        magnitudes: float64 two-particle closed forms of viscosity
+ *                        and of cohesion + curvature (both C(r) branches, K_ij) and a
+ *                        brute-force cluster sum from the published formulas
+ *                        (tests/test_oracle_forces.py)
  *   integrate         -> free-fall closed form
  *   update_pressure   -> dv = max(0, rho* - rho0) and p += delta*dv by hand on
  *                        a two-particle scene; boundary-mode pass with init term
- *   pressure_force    -> fluid-only scenes: sum of forces ~ 0 (antisymmetry),
- *                        two-particle closed form
+ *   pressure_force    -> fluid-only scenes: sum of forces ~ 0 (antisymmetry);
+ *                        magnitudes: float64 brute force of the spiky-gradient sum
+ *                        fluid-fluid, and fluid-boundary with rho_b = rho0 (with and
+ *                        without the boundary-boundary pass) (tests/test_oracle_forces.py)
  *   correction loop   -> min/max iteration semantics on tiny scenes
  * OpenCL built-ins are restated as IEEE operations: distance()/fast_length() as
  * sqrt(dot), `/` correctly rounded, no FP contraction (built with
