@@ -205,8 +205,8 @@ int pcisph_field_size(const pcisph_solver* s, int field, size_t* out_count);
 /* Copy a field in or out in the reference layout.  `count` must equal
  * pcisph_field_size (PCISPH_ERR_SIZE otherwise).  Host pointers; downloads wait
  * for the step to finish.  Uploading PCISPH_BOUNDARY_POSITIONS schedules a
- * boundary re-sort at the next step.  Sort products and neighbour counts are
- * read-only. */
+ * boundary re-sort at the next step (in a batch: of every scene's walls, in one
+ * sort).  Sort products and neighbour counts are read-only. */
 int pcisph_upload(pcisph_solver* s, int field, const void* host, size_t count);
 int pcisph_download(pcisph_solver* s, int field, void* host, size_t count);
 /* Same, with a device pointer on the solver's device (ordered on its stream). */
@@ -316,14 +316,20 @@ int pcisph_step_scenes(pcisph_solver* s, pcisph_step_stats* stats, size_t count)
  * in the reference, whose boundary never moves. */
 int pcisph_translate_boundary(pcisph_solver* s, float dx, float dy, float dz);
 /* Every scene's walls in one launch: scene k moves by shifts[3k .. 3k+2]
- * (count = 3 * scene count); scenes with a non-zero shift are re-sorted next step. */
+ * (count = 3 * scene count).  If any shift is non-zero the next step re-sorts the walls
+ * of the batch, all scenes in one sort: an unmoved scene's walls keep their positions
+ * and order, and its BOUNDARY_SRC_LOCATIONS become the identity (the permutation of
+ * sorting walls that are already sorted), as after an upload of its walls. */
 int pcisph_translate_boundaries(pcisph_solver* s, const float* shifts, size_t count);
 
-/* Autotuner: times `trial_steps` steps of the current scene for each threads-per-block
+/* Autotuner: times `trial_steps` steps of the current scene(s) for each threads-per-block
  * choice (64..512) and radix digit width (5..8 bits) from the same saved starting state,
  * keeps the fastest in the solver options (and returns them in `chosen` if non-NULL),
- * and restores the state.  No counterpart in the reference (clogs has its own offline
- * tuner, exec_clogs_tune.bat). */
+ * and restores the state (particles, pending boundary sorts, warm-start eligibility,
+ * sample collection; the options too if it fails).  It takes the snapshot slot of
+ * pcisph_snapshot: a snapshot taken before the call is replaced by the state at the
+ * call.  No counterpart in the reference (clogs has its own offline tuner,
+ * exec_clogs_tune.bat). */
 int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen);
 
 /* Device-side checkpoint of every scene's state at a step boundary: fluid positions and

@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -2094,22 +2095,46 @@ int pcisph_set_warm_start(pcisph_solver* s, int enable) {
 // Times `trial_steps` steps of the current scene for each block size of the particle
 // kernels and each radix digit width, from the same starting state each time (a
 // device snapshot restored after every trial), then keeps the fastest pair in the
-// solver options.  The state after the call is the state before it.
+// solver options.  The state after the call is the state before it: particle state,
+// pending boundary sorts, each scene's warm-start eligibility (its capped flag) and
+// sample collection; on an error the options are the caller's again too.
 int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen) {
     return guarded(s, [&] {
         if (!s->total_fluid()) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "no particles to tune on");
         if (!trial_steps) trial_steps = 3;
         apply_param_changes(s);
         // the boundary sort a pending update would do is part of every trial's first step
-        std::vector<bool> boundary_pending;
-        for (auto& sc : s->scenes) boundary_pending.push_back(sc.boundary_updated);
+        std::vector<bool> boundary_pending, capped;
+        for (auto& sc : s->scenes) {
+            boundary_pending.push_back(sc.boundary_updated);
+            capped.push_back(sc.capped);
+        }
         auto reset_boundary = [&] {
-            for (size_t i = 0; i < s->scenes.size(); ++i) s->scenes[i].boundary_updated = boundary_pending[i];
+            for (size_t i = 0; i < s->scenes.size(); ++i) {
+                s->scenes[i].boundary_updated = boundary_pending[i];
+                s->scenes[i].capped = capped[i];
+            }
         };
         s->eng->snapshot();
         const pcisph_solver_options base = s->opt;
-        const uint32_t collect = s->collect_stride;
+        const uint32_t collect = s->collect_stride, phase = s->collect_phase;
         s->collect_stride = 0;
+        // puts the caller's state back however the trials end
+        struct Guard {
+            std::function<void()> fn;
+            ~Guard() {
+                try {
+                    fn();
+                } catch (...) {
+                }
+            }
+        } guard{[&] {
+            s->opt = base;
+            s->collect_stride = collect;
+            s->collect_phase = phase;
+            reset_boundary();
+            s->eng->restore();
+        }};
         double best = 1e300;
         pcisph_solver_options best_opt = base;
         auto trial = [&](uint32_t block, uint32_t digit) {
@@ -2134,10 +2159,9 @@ int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_option
         for (uint32_t block : {64u, 128u, 256u, 512u}) trial(block, 8u);
         const uint32_t block = best_opt.block_size;
         for (uint32_t digit : {5u, 6u, 7u}) trial(block, digit);
+        guard.fn();
+        guard.fn = [] {};
         s->opt = best_opt;
-        s->collect_stride = collect;
-        s->eng->restore();
-        reset_boundary();
         if (chosen) *chosen = best_opt;
     });
 }
