@@ -221,6 +221,8 @@ def main() -> int:
                          "steps in, 100 steps each), batch_256 / batch_256_warm (256 scenes of 1,024..65,536 "
                          "particles: dam breaks, sphere drops, sloshing tanks; plain / trained network)")
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--work-order", type=int, default=None,
+                    help="solver option work_order for the single-scene forms (default: the solver's)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "pcisph_amd", "data", "warm_start.npy"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device five times (the reference kernel for scaling "
@@ -284,7 +286,7 @@ def main() -> int:
             sc = scenes.sloshing_tank(0.5, 0.8, getattr(scenes, args.workload)()) \
                 if form.startswith("sloshing") else scene
             fluid = pcisph_amd.Fluid(device=0, precision=prec)
-            fluid.load_scene(sc)
+            fluid.load_scene(sc, **({} if args.work_order is None else {"work_order": args.work_order}))
             if form in ("warm_start", "sloshing_warm"):
                 if not os.path.exists(args.weights):
                     print(f"bench: no trained network at {args.weights}; warm_start skipped", file=sys.stderr)

@@ -128,8 +128,15 @@ typedef struct pcisph_solver_options {
                                        passes with 256-thread workgroups stage each
                                        workgroup's neighbour buckets in LDS and walk them
                                        there (measured slower on the dam breaks); bit 3: the
-                                       Morton layout (bit 0) at any size.  Default 3 (bits 0
-                                       and 1). */
+                                       Morton layout (bit 0) at any size; bits 4 and 5: the
+                                       FP32 fluid pressure passes take their candidates from a
+                                       per-step LDS staging (each workgroup's particles in the
+                                       Morton order of their own cells, the bucket ranges of
+                                       their rows copied once per pass and walked in the
+                                       reference's order: identical results) -- automatically
+                                       when no scene's pressure passes may walk neighbour
+                                       lists in the step, always with bit 4, never with bit 5.
+                                       Default 3 (bits 0 and 1). */
     uint32_t reserved;
 } pcisph_solver_options;
 

@@ -216,6 +216,11 @@ size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits) {
     return words + scan_partial_words(words);  // histograms, then the scan's chunk totals
 }
 
+int radix_sort_result(uint32_t n, uint32_t key_bits, uint32_t digit_bits) {
+    if (n <= 1 || key_bits == 0) return 0;
+    return (int)(((key_bits + digit_bits - 1) / digit_bits) & 1u);
+}
+
 int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
                uint32_t digit_bits, uint32_t* scratch) {
     if (n <= 1 || key_bits == 0) return 0;

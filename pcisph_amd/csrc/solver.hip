@@ -21,6 +21,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pcisph.h"
@@ -418,6 +419,14 @@ class Engine final : public EngineBase {
         dfree(A_.bgrid);
         dfree(ext_dev_);
         dfree(sort_scratch_);
+        for (int k = 0; k < 2; ++k) {
+            dfree(stage_keys_[k]);
+            dfree(stage_vals_[k]);
+        }
+        dfree(stage_rows_);
+        dfree(stage_src_);
+        dfree(stage_info_);
+        stage_n_ = stage_tiles_ = 0;
     }
 
     // New arrays of `count` elements holding each scene's old contents (at most the
@@ -801,9 +810,11 @@ class Engine final : public EngineBase {
         const uint32_t want = block | (xcd ? 0x80000000u : 0u);
         if (blocks_for_ == want) return;
         std::vector<BlockRange> t;
+        // fluid ranges carry their index in this order (the staging index of sph_staged.h)
         for (uint32_t s = 0; s < n_.size(); ++s)
             for (uint32_t b = f_off_[s]; b < f_off_[s + 1]; b += block)
-                t.push_back({s, b, std::min(b + block, f_off_[s + 1]), b == f_off_[s] ? kRangeFirstOfScene : 0u});
+                t.push_back({s, b, std::min(b + block, f_off_[s + 1]),
+                             (b == f_off_[s] ? kRangeFirstOfScene : 0u) | ((uint32_t)t.size() << kRangeTileShift)});
         const uint32_t fluid_blocks = (uint32_t)t.size();
         if (xcd && fluid_blocks > 8) {
             constexpr uint32_t kXcds = 8, kChunk = 16, kGroup = kXcds * kChunk;
@@ -980,11 +991,12 @@ class Engine final : public EngineBase {
     // Every value a recorded launch bakes in; graphs are re-captured when it changes.
     std::vector<uint64_t> launch_key(const Launch& L, bool fast, bool warm, bool features, uint32_t digit,
                                      uint32_t key_bits, uint32_t maxit, uint32_t chunk, uint32_t layout_bits,
-                                     uint32_t morton_bits, const float* mlp) const {
+                                     uint32_t morton_bits, const float* mlp, bool staged) const {
         std::vector<uint64_t> k;
         auto put = [&](uint64_t v) { k.push_back(v); };
         auto ptr = [&](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
         put(L.block), put(fast), put(warm), put(features), put(digit), put(key_bits), put(maxit), put(chunk);
+        put(staged), ptr(A_.stage.work), ptr(A_.stage.rows), ptr(A_.stage.src), ptr(A_.stage.info);
         put(layout_bits), put(morton_bits);
         put(A_.tile_pressure);
         put(A_.n), put(A_.nb), put(A_.B);
@@ -1044,6 +1056,21 @@ class Engine final : public EngineBase {
                             A_.n, A_.nb, 0, A_.box_cells ? A_.fgrid : nullptr,
                             A_.box_cells ? A_.bgrid : nullptr};
         A_.tile_pressure = (o.work_order & 4) != 0;
+        // The FP32 fluid pressure pass from the step's LDS staging: where no scene's
+        // pressure passes may walk their lists this step (work_order bit 4 forces it,
+        // bit 5 switches it off)
+        bool all_buckets = true;
+        for (size_t s = 0; s < scene_host_.size(); ++s)
+            all_buckets = all_buckets && (!A_.NL.on || !(scene_host_[s].disp_limit > 0.f) || !n_[s]);
+        const bool staged = std::is_same<R, float>::value && A_.n > 0 && !(o.work_order & 32) &&
+                            ((o.work_order & 16) || all_buckets);
+        const uint32_t stage_bits = 3 * obits + sbits;
+        A_.staged_pressure = staged;
+        if (staged) {
+            ensure_stage_buffers(fluid_blocks_);
+            A_.stage = StageMaps{stage_vals_[radix_sort_result(A_.n, stage_bits, digit)], stage_rows_, stage_src_,
+                                 stage_info_};
+        }
         if (features) ensure_warm_buffers(warm);
         D_.disp_next = A_.NL.on ? A_.NL.disp[1] : nullptr;
         D_.buckets = A_.NL.buckets;
@@ -1053,7 +1080,7 @@ class Engine final : public EngineBase {
         const uint32_t chunk = prof_.begin_step() ? 0u : o.graph_iterations;
         if (chunk) {
             std::vector<uint64_t> key =
-                launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params);
+                launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params, staged);
             if (key != graph_key_) {
                 drop_graphs();
                 graph_key_ = key;
@@ -1122,6 +1149,9 @@ class Engine final : public EngineBase {
                                            A_.vel_in, A_.pos, A_.vel, A_.cells);
                 }
                 if (A_.box_cells) DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.cells, A_.fgrid);
+                if (staged)
+                    DataLaunch<R>::stage_setup(L, A_, obits, stage_bits, digit, sort_scratch_, stage_keys_,
+                                               stage_vals_);
             });
             prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
             prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, A_); });
@@ -1328,6 +1358,23 @@ class Engine final : public EngineBase {
 
     static constexpr uint32_t kLayoutMinParticles = 1u << 17;
 
+    // Buffers of the pressure pass's staging for `tiles` fluid ranges.
+    void ensure_stage_buffers(uint32_t tiles) {
+        if (stage_n_ != A_.n || !stage_rows_) {
+            for (int k = 0; k < 2; ++k) {
+                dalloc(stage_keys_[k], std::max(A_.n, 1u), stream_);
+                dalloc(stage_vals_[k], std::max(A_.n, 1u), stream_);
+            }
+            dalloc(stage_rows_, 9 * (size_t)std::max(A_.n, 1u), stream_);
+            stage_n_ = A_.n;
+        }
+        if (stage_tiles_ < tiles || !stage_src_) {
+            dalloc(stage_src_, (size_t)std::max(tiles, 1u) * kStageCap, stream_);
+            dalloc(stage_info_, std::max(tiles, 1u), stream_);
+            stage_tiles_ = tiles;
+        }
+    }
+
     // Buffers of the Morton layout: its sort, the bucket codes, the reference's
     // source locations.  The slot maps canon_ / stor_ live with the fluid arrays.
     void ensure_order_buffers() {
@@ -1435,6 +1482,14 @@ class Engine final : public EngineBase {
     float list_skin_ = 0.f;
     std::vector<uint8_t> near_;          // per scene: lists of radius (1 + kNearSkin) h this step
     std::vector<uint32_t> near_steps_;   // steps since the scene went to near lists
+    // staging of the FP32 pressure pass (sph_staged.h): work-order sort, row
+    // descriptors [9][n], staging maps [fluid ranges][kStageCap], staged counts
+    uint32_t* stage_keys_[2] = {nullptr, nullptr};
+    uint32_t* stage_vals_[2] = {nullptr, nullptr};
+    uint32_t* stage_rows_ = nullptr;
+    uint32_t* stage_src_ = nullptr;
+    uint32_t* stage_info_ = nullptr;
+    uint32_t stage_n_ = 0, stage_tiles_ = 0;
     uint32_t* okeys_[2] = {nullptr, nullptr};  // layout sort: bucket codes
     uint32_t* ovals_[2] = {nullptr, nullptr};  // and canonical indices (the result is the layout)
     uint32_t* canon_ = nullptr;                // slot -> canonical index of the public fluid arrays
@@ -1679,7 +1734,7 @@ int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "sort_digit_bits must be in 4..8");
         if (!(opt->neighbor_skin >= 0.f && opt->neighbor_skin <= 2.f))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "neighbor_skin must be in [0, 2]");
-        if (opt->work_order > 15) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..15");
+        if (opt->work_order > 63) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..63");
         s->opt = *opt;
     });
 }

@@ -267,6 +267,18 @@ void DataLaunch<R>::hash(const Launch& L, const Batch<R>& B, bool boundary, cons
 }
 
 template <typename R>
+void DataLaunch<R>::stage_setup(const Launch& L, const DeviceArrays<R>& A, uint32_t bits, uint32_t key_bits,
+                                uint32_t digit_bits, uint32_t* scratch, uint32_t* keys[2], uint32_t* vals[2]) {
+    const Batch<R>& B = A.batch;
+    if (!B.fluid_blocks) return;
+    hipLaunchKernelGGL((k_stage_codes<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B.scenes, B.blocks,
+                       A.pos, bits, keys[0], vals[0]);
+    radix_sort(L, keys, vals, B.n_total, key_bits, digit_bits, scratch);
+    hipLaunchKernelGGL((k_stage_setup<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B, A.cells, A.pos,
+                       A.stage);
+}
+
+template <typename R>
 void DataLaunch<R>::warm_apply(const Launch& L, const DeviceArrays<R>& A) {
     if (!A.batch.fluid_blocks) return;
     hipLaunchKernelGGL((k_warm_apply<R>), dim3(A.batch.fluid_blocks), dim3(L.block), 0, L.stream, A.batch, A.xr,

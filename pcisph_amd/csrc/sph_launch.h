@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "sph_kernels.h"
+#include "sph_staged.h"
 
 namespace pcisph {
 
@@ -49,6 +50,9 @@ struct DeviceArrays {
     uint32_t box_cells = 0;
     NeighborLists NL;  // on == 0: no lists this step
     bool tile_pressure = false;  // FP32 fluid pressure passes from LDS tiles (256-thread workgroups)
+    // FP32 fluid pressure passes from the per-step LDS staging (sph_staged.h)
+    bool staged_pressure = false;
+    StageMaps stage{};
 };
 
 // `block` must be the workgroup size the batch's block table was built for.
@@ -135,6 +139,12 @@ struct DataLaunch {
     // cell extents of every scene's fluid and boundary particles into ext[12 S]
     // (k_extents; ext prepared by the caller)
     static void extents(const Launch& L, const Batch<R>& B, const V4<R>* pos, const V4<R>* bpos, int* ext);
+    // the per-step staging of the pressure pass (sph_staged.h): the work order -- the
+    // fluid slots sorted stably by the Morton code of their own cell (`bits` per axis,
+    // scene index above) through keys/vals -- then each fluid range's staging map and
+    // row descriptors into A.stage (whose `work` must be vals[radix_sort_result(...)])
+    static void stage_setup(const Launch& L, const DeviceArrays<R>& A, uint32_t bits, uint32_t key_bits,
+                            uint32_t digit_bits, uint32_t* scratch, uint32_t* keys[2], uint32_t* vals[2]);
 };
 
 void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);
@@ -154,5 +164,7 @@ void fill_u64(const Launch& L, unsigned long long* p, size_t n, unsigned long lo
 size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits);
 int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
                uint32_t digit_bits, uint32_t* scratch);
+// The buffer index radix_sort returns for these sizes (known before it is enqueued).
+int radix_sort_result(uint32_t n, uint32_t key_bits, uint32_t digit_bits);
 
 }  // namespace pcisph

@@ -1,0 +1,357 @@
+// sph_staged.h - the fluid pressure pass with its candidates staged in LDS.
+//
+// The pressure pass (update_pressure, PCISPH/data/kernels/pcisph.cl:212-263) runs in
+// every correction iteration and gathers, per fluid particle, the predicted position
+// of every particle in the 27 buckets around its cell (~200 candidates at h = 2
+// spacings): one divergent 16-byte load per candidate per lane.  Here the bookkeeping
+// that lets a workgroup share those loads is done once per step, and each iteration
+// only copies and walks:
+//
+//  * work order: fluid particles are taken in the Morton order of their own cell
+//    (a stable sort of the slots by that code), so a workgroup's particles occupy a
+//    compact block of cells even where a bucket holds particles of two far cells (hash
+//    collisions place them with their bucket, not with their cell);
+//  * staging map (per workgroup): for each (y, z) line of its box of cells, the run of
+//    x-adjacent cells its particles' rows on that line cover; the cells' bucket ranges
+//    (the reference's buckets, foreign particles included) are staged one after
+//    another, so the three buckets of any particle's row are consecutive in LDS;
+//  * row descriptors (per particle): LDS start and length of its nine (dz, dy) rows.
+//
+// Each iteration a workgroup copies its staged predicted positions into LDS (coalesced
+// within each bucket) and every lane walks its nine rows there: the reference's
+// candidates in the reference's order (rows dz-major then dy, x fastest within a row,
+// ascending index within a bucket), so sums are unchanged bit for bit.  Boundary
+// candidates are walked from global memory as in k_pressure.  A workgroup whose
+// staging does not fit (kStageCap candidates, kStageMaxCells cells, kStageLines lines,
+// cell coordinates of magnitude 2^29 or more) walks global memory instead.
+#pragma once
+
+#include "sph_kernels.h"
+
+namespace pcisph {
+
+constexpr uint32_t kRangeTileShift = 8;      // fluid ranges: their workgroup's staging index above the flag bits
+constexpr uint32_t kStageCap = 2048;         // staged candidates per workgroup (x, y, z: 24 KB of LDS)
+constexpr uint32_t kStageMaxCells = 2048;    // staged cells per workgroup
+constexpr uint32_t kStageLines = 1024;       // (y, z) lines of a workgroup's box
+constexpr uint32_t kStageFallback = 0xFFFFFFFFu;
+constexpr int32_t kStageCoordLimit = 1 << 29;
+constexpr int kCopyBatch = 8;   // staging loads in flight per thread
+constexpr int kRangeBatch = 4;  // cell-range lookups in flight per thread (setup)
+
+struct StageMaps {
+    const uint32_t* work;  // [n_total] work index -> fluid slot (per-step Morton order of own cells)
+    uint32_t* rows;        // [9][n_total] per work index: row r's LDS start | length << 16
+    uint32_t* src;         // [workgroups][kStageCap] slot of each staged candidate
+    uint32_t* info;        // [workgroups] staged candidate count, or kStageFallback
+};
+
+// The bucket range of cell (x, y, z) as for_each_range_row reads it (grid flags removed).
+__device__ __forceinline__ uint2 cell_range(const CellTable& T, uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t rx = x - T.lo[0], ry = y - T.lo[1], rz = z - T.lo[2];
+    if (rx < T.dim[0] && ry < T.dim[1] && rz < T.dim[2]) {
+        uint2 r = T.grid[rx + T.dim[0] * (ry + T.dim[1] * rz)];
+        r.x &= ~kFarBucket;
+        r.y &= ~kQuietCell;
+        return r;
+    }
+    return T.cells[hash_key(x, y, z, T.g)];
+}
+
+// Exclusive workgroup scan of two counters (any multiple of 64 threads up to
+// kMaxBlockThreads); `total` gets the sums.
+__device__ __forceinline__ uint2 block_scan2(uint32_t a, uint32_t b, uint2& total, uint32_t (*part)[kMaxBlockThreads / 64]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = (int)(blockDim.x >> 6);
+    uint32_t xa = a, xb = b;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ya = __shfl_up(xa, off), yb = __shfl_up(xb, off);
+        if (lane >= off) xa += ya, xb += yb;
+    }
+    if (lane == 63) part[0][wave] = xa, part[1][wave] = xb;
+    __syncthreads();
+    uint32_t ba = 0, bb = 0, ta = 0, tb = 0;
+    for (int w = 0; w < waves; ++w) {
+        if (w < wave) ba += part[0][w], bb += part[1][w];
+        ta += part[0][w], tb += part[1][w];
+    }
+    __syncthreads();  // part may be reused by the caller
+    total = make_uint2(ta, tb);
+    return make_uint2(ba + xa - a, bb + xb - b);
+}
+
+// Code of a fluid particle's own cell: scene index above the Morton code of the cell
+// coordinates modulo 2^bits (the Morton layout's code of buckets, taken per particle).
+__device__ __forceinline__ uint32_t stage_spread3(uint32_t v, uint32_t bits) {
+    uint32_t out = 0;
+    for (uint32_t k = 0; k < bits; ++k) out |= ((v >> k) & 1u) << (3 * k);
+    return out;
+}
+
+template <typename R>
+__global__ void k_stage_codes(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
+                              const V4<R>* __restrict__ pos, uint32_t bits, uint32_t* __restrict__ keys,
+                              uint32_t* __restrict__ vals) {
+    const BlockRange rg = blocks[blockIdx.x];
+    const uint32_t i = rg.begin + threadIdx.x;
+    if (i >= rg.end) return;
+    const int3 q = cell_of4<R>(pos[i], scenes[rg.scene].cell);
+    keys[i] = (rg.scene << (3 * bits)) | stage_spread3((uint32_t)q.x, bits) |
+              (stage_spread3((uint32_t)q.y, bits) << 1) | (stage_spread3((uint32_t)q.z, bits) << 2);
+    vals[i] = i;
+}
+
+// Per step, one workgroup per fluid range of the block table (work indices
+// [begin, end)): the staging map and the row descriptors of its particles, from the
+// positions of the step (its buckets) and the fluid cell table.  Lines are the (y, z)
+// rows of cells of the workgroup's bounding box of particle cells grown by one cell;
+// each line stages the run of x-adjacent cells from the lowest x any particle's row
+// on that line needs to the highest, so each particle's row of three cells is one run
+// of LDS.
+template <typename R>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, const uint2* __restrict__ fcells,
+                                                                  const V4<R>* __restrict__ pos, StageMaps S) {
+    __shared__ int s_lx[kStageLines], s_hx[kStageLines];  // x extent of each line
+    __shared__ uint32_t s_cell0[kStageLines + 1];         // first staged cell of each line
+    __shared__ uint32_t s_start[kStageMaxCells];          // bucket start of each staged cell
+    __shared__ uint32_t s_off[kStageMaxCells + 1];        // staged offset of each cell (count before the scan)
+    __shared__ uint32_t s_part[2][kMaxBlockThreads / 64];
+    __shared__ int s_box[6][kMaxBlockThreads / 64];
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const uint32_t tile = rg.flags >> kRangeTileShift;
+    const SphParams<R>& P = B.scenes[rg.scene];
+    const CellTable FT = cell_table(P, fcells, B.fgrid);
+    const uint32_t k = rg.begin + threadIdx.x;
+    const bool act = k < rg.end;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = (int)(blockDim.x >> 6);
+    int3 c = make_int3(0, 0, 0);
+    if (act) c = cell_of4<R>(pos[S.work[k]], P.cell);
+    // 1. the box of the workgroup's cells
+    int lo[3] = {act ? c.x : INT32_MAX, act ? c.y : INT32_MAX, act ? c.z : INT32_MAX};
+    int hi[3] = {act ? c.x : INT32_MIN, act ? c.y : INT32_MIN, act ? c.z : INT32_MIN};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], off));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], off));
+        }
+        if (lane == 0) {
+            s_box[a][wave] = lo[a];
+            s_box[3 + a][wave] = hi[a];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        for (int w = 0; w < waves; ++w) {
+            lo[a] = min(lo[a], s_box[a][w]);
+            hi[a] = max(hi[a], s_box[3 + a][w]);
+        }
+    bool fits = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fits = fits && lo[a] > -kStageCoordLimit && hi[a] < kStageCoordLimit;
+    const uint32_t by = fits ? (uint32_t)(hi[1] - lo[1] + 3) : 0u, bz = fits ? (uint32_t)(hi[2] - lo[2] + 3) : 0u;
+    const uint64_t lines = (uint64_t)by * bz;
+    if (!fits || lines > kStageLines) {
+        if (threadIdx.x == 0) S.info[tile] = kStageFallback;
+        return;
+    }
+    const uint32_t nl = (uint32_t)lines;
+    const int y0 = lo[1] - 1, z0 = lo[2] - 1;
+    for (uint32_t l = threadIdx.x; l < nl; l += blockDim.x) {
+        s_lx[l] = INT32_MAX;
+        s_hx[l] = INT32_MIN;
+    }
+    __syncthreads();
+    // 2. x extent of each line: the first lane of each run of one cell (a cell's
+    // particles are adjacent in the work order) records the rows of all of them
+    const int px = __shfl_up(c.x, 1), py = __shfl_up(c.y, 1), pz = __shfl_up(c.z, 1);
+    if (act && (lane == 0 || px != c.x || py != c.y || pz != c.z)) {
+        for (int row = 0; row < 9; ++row) {
+            const uint32_t l = (uint32_t)(c.z + row / 3 - 1 - z0) * by + (uint32_t)(c.y + row % 3 - 1 - y0);
+            atomicMin(&s_lx[l], c.x - 1);
+            atomicMax(&s_hx[l], c.x + 1);
+        }
+    }
+    __syncthreads();
+    // 3. first staged cell of each line: a contiguous chunk of lines per thread, scanned
+    {
+        const uint32_t chunk = (nl + blockDim.x - 1) / blockDim.x;
+        const uint32_t l0 = min(threadIdx.x * chunk, nl), l1 = min(l0 + chunk, nl);
+        uint32_t sum = 0;
+        for (uint32_t l = l0; l < l1; ++l) sum += s_hx[l] >= s_lx[l] ? (uint32_t)(s_hx[l] - s_lx[l] + 1) : 0u;
+        uint2 tot;
+        const uint2 base = block_scan2(sum, 0u, tot, s_part);
+        uint32_t run = base.x;
+        for (uint32_t l = l0; l < l1; ++l) {
+            s_cell0[l] = run;
+            run += s_hx[l] >= s_lx[l] ? (uint32_t)(s_hx[l] - s_lx[l] + 1) : 0u;
+        }
+        if (threadIdx.x == 0) s_cell0[nl] = tot.x;
+        if (tot.x > kStageMaxCells) {
+            if (threadIdx.x == 0) S.info[tile] = kStageFallback;
+            return;
+        }
+    }
+    __syncthreads();
+    const uint32_t ncells = s_cell0[nl];
+    // 4. each staged cell's bucket range (kRangeBatch lookups in flight per thread)
+    for (uint32_t q0 = threadIdx.x; q0 < ncells; q0 += kRangeBatch * blockDim.x) {
+        uint2 r[kRangeBatch];
+#pragma unroll
+        for (int u = 0; u < kRangeBatch; ++u) {
+            const uint32_t q = q0 + u * blockDim.x;
+            r[u] = make_uint2(0u, 0u);
+            if (q >= ncells) continue;
+            uint32_t a = 0, b = nl;  // the last line whose first cell is <= q (a non-empty one)
+            while (b - a > 1) {
+                const uint32_t mid = (a + b) >> 1;
+                if (s_cell0[mid] <= q) a = mid;
+                else b = mid;
+            }
+            const int x = s_lx[a] + (int)(q - s_cell0[a]);
+            r[u] = cell_range(FT, (uint32_t)x, (uint32_t)(y0 + (int)(a % by)), (uint32_t)(z0 + (int)(a / by)));
+        }
+#pragma unroll
+        for (int u = 0; u < kRangeBatch; ++u) {
+            const uint32_t q = q0 + u * blockDim.x;
+            if (q >= ncells) continue;
+            s_start[q] = r[u].x;
+            s_off[q] = r[u].y - r[u].x;
+        }
+    }
+    __syncthreads();
+    // 5. staged offsets: a contiguous chunk of cells per thread, scanned
+    const uint32_t chunk = (ncells + blockDim.x - 1) / blockDim.x;
+    const uint32_t q0 = min(threadIdx.x * chunk, ncells), q1 = min(q0 + chunk, ncells);
+    uint32_t sum = 0;
+    for (uint32_t q = q0; q < q1; ++q) sum += s_off[q];
+    uint2 tot;
+    const uint2 base = block_scan2(sum, 0u, tot, s_part);
+    {
+        uint32_t run = base.x;
+        for (uint32_t q = q0; q < q1; ++q) {
+            const uint32_t cnt = s_off[q];
+            s_off[q] = run;
+            run += cnt;
+        }
+    }
+    const uint32_t total = tot.x;
+    if (threadIdx.x == 0) s_off[ncells] = total;
+    if (total > kStageCap) {
+        if (threadIdx.x == 0) S.info[tile] = kStageFallback;
+        return;
+    }
+    __syncthreads();
+    // 6. the staging map and the rows
+    uint32_t* src = S.src + (size_t)tile * kStageCap;
+    for (uint32_t q = threadIdx.x; q < ncells; q += blockDim.x) {
+        const uint32_t o = s_off[q], n = s_off[q + 1] - o, b = s_start[q];
+        for (uint32_t e = 0; e < n; ++e) src[o + e] = b + e;
+    }
+    if (act) {
+        for (int row = 0; row < 9; ++row) {
+            const uint32_t l = (uint32_t)(c.z + row / 3 - 1 - z0) * by + (uint32_t)(c.y + row % 3 - 1 - y0);
+            const uint32_t q = s_cell0[l] + (uint32_t)(c.x - 1 - s_lx[l]);
+            const uint32_t a = s_off[q], e = s_off[q + 3];
+            S.rows[(size_t)row * B.n_total + k] = a | ((e - a) << 16);
+        }
+    }
+    if (threadIdx.x == 0) S.info[tile] = total;
+}
+
+// update_pressure (pcisph.cl:212-263) on the fluid ranges, candidates from the
+// workgroup's LDS staging (see above); the same results as k_pressure's bucket walk.
+// Fluid ranges only; `blocks` entries carry their staging index (kRangeTileShift).
+template <bool FAST>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
+    Batch<float> B, uint32_t iter, const uint2* __restrict__ bcells, const float4* __restrict__ bpos,
+    const uint2* __restrict__ fcells, const float4* __restrict__ xr, const float4* __restrict__ xp,
+    float* __restrict__ dv, float* __restrict__ p, float4* __restrict__ xf, StageMaps S, NeighborLists NL) {
+    using O = Ops<float, FAST>;
+    __shared__ float s_x[kStageCap], s_y[kStageCap], s_z[kStageCap];
+    __shared__ unsigned long long lds[2][8];
+    if (NL.on) clear_table(NL.disp[(iter + 1) & 1], NL.buckets);  // for the next prediction
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<float> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
+    if (iteration_skipped<float>(st, iter, P)) return;
+    const uint32_t tile = rg.flags >> kRangeTileShift;
+    const uint32_t total = S.info[tile];
+    const bool staged = total != kStageFallback;
+    if (staged) {
+        // kCopyBatch independent loads in flight per thread (indices, then positions)
+        const uint32_t* src = S.src + (size_t)tile * kStageCap;
+        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kCopyBatch * blockDim.x) {
+            uint32_t j[kCopyBatch];
+            float4 v[kCopyBatch];
+#pragma unroll
+            for (int u = 0; u < kCopyBatch; ++u) {
+                const uint32_t q = q0 + u * blockDim.x;
+                j[u] = src[q < total ? q : q0];
+            }
+#pragma unroll
+            for (int u = 0; u < kCopyBatch; ++u) v[u] = xp[j[u]];
+#pragma unroll
+            for (int u = 0; u < kCopyBatch; ++u) {
+                const uint32_t q = q0 + u * blockDim.x;
+                if (q < total) {
+                    s_x[q] = v[u].x;
+                    s_y[q] = v[u].y;
+                    s_z[q] = v[u].z;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t k = rg.begin + threadIdx.x;
+    unsigned long long m[2] = {0, 0};  // max dv, max expansion
+    if (k < rg.end) {
+        const uint32_t i = S.work[k];
+        const float4 self = xr[i], pr = xp[i];
+        const int3 c = cell_of4<float>(self, P.cell);
+        float pd = 0.f;
+        auto add = [&](const float4& o, uint32_t) {
+            float dx, dy, dz;
+            pd += poly6<float>(dist2<float>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
+        };
+        if (P.nb) {
+            CellTable BT = cell_table(P, bcells, B.bgrid);
+            BT.prune = far_prunable<float>(displacement<float>(pr, self), (float)P.h, P.cell);
+            if (!(BT.prune && quiet_cell(BT, c)))
+                walk_buckets<float>(BT, c, [&](uint32_t j) { return bpos[j]; }, add);
+        }
+        if (staged) {
+            uint32_t d[9];
+#pragma unroll
+            for (int row = 0; row < 9; ++row) d[row] = S.rows[(size_t)row * B.n_total + k];
+#pragma unroll 1
+            for (int row = 0; row < 9; ++row) {
+                const uint32_t a = d[row] & 0xFFFFu;
+                gather_range(a, a + (d[row] >> 16), [&](uint32_t j) { return make_float4(s_x[j], s_y[j], s_z[j], 0.f); },
+                             add);
+            }
+        } else {
+            walk_buckets<float>(cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xp[j]; }, add);
+        }
+        pd *= P.density_scale;
+        const float v = fmaxf(0.f, pd - P.rho0);
+        // expansion counts only where the step began at or above rest density
+        if (self.w >= P.rho0) m[1] = Bits<float>::of(fmaxf(0.f, P.rho0 - pd));
+        dv[i] = v;
+        float pi = p[i];
+        if (v != 0.f) {
+            pi += v * P.delta;
+            p[i] = pi;
+        }
+        float4 out = self;
+        out.w = O::div(pi, self.w * self.w);
+        xf[i] = out;
+        m[0] = Bits<float>::of(v);
+    }
+    unsigned long long* const dst[2] = {&st->max_dv[iter], &st->max_exp[iter]};
+    block_max_to<2>(m, dst, lds);
+}
+
+}  // namespace pcisph

@@ -340,3 +340,65 @@ def test_autotune_restores_warm_start_eligibility(gpu):
         for key in STAT_KEYS + ("warm_start",):
             assert x[key] == y[key], (key, x[key], y[key])
     assert_same_state(a, b, tag="autotune warm")
+
+
+# ---------------------------------------------------------------- staged pressure pass
+STAGED = 3 | 16  # default work order plus the LDS-staged FP32 pressure pass in every step
+
+
+@pytest.mark.parametrize("scene_idx", [0, 1])
+@pytest.mark.parametrize("layout", [0, 8])
+def test_staged_pressure_bit_exact(gpu, oracle_lib, scene_idx, layout):
+    """The pressure pass from the per-step LDS staging (work order of own cells, rows
+    walked in LDS) leaves the oracle's bits, with and without the Morton layout."""
+    scene = small_scenes()[scene_idx]
+    print(f"case staged pressure {scene.name} layout={layout}", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 3, max_iterations=20)
+    sg = run(g, scene, 3, max_iterations=20, work_order=STAGED | layout)
+    assert_same_stats(sg, so, "staged")
+    assert_same_state(g, o, tag="staged")
+
+
+@pytest.mark.parametrize("block", [64, 256, 512])
+def test_staged_pressure_block_sizes(gpu, oracle_lib, block):
+    scene = small_scenes()[1]
+    print(f"case staged pressure block {block}", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 2, max_iterations=20)
+    sg = run(g, scene, 2, max_iterations=20, work_order=STAGED, block_size=block)
+    assert_same_stats(sg, so, f"staged block {block}")
+    assert_same_state(g, o, tag=f"staged block {block}")
+
+
+def test_1m_staged_two_steps_bit_exact(gpu, oracle_1m):
+    scene, ref = oracle_1m
+    print("case 1m staged pressure ieee, 2 steps", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    g.load_scene(scene, min_iterations=3, max_iterations=3, work_order=STAGED)
+    for k, (st_o, snap_o) in enumerate(ref):
+        st_g = g.step()
+        assert_same_stats([st_g], [st_o], f"staged step {k + 1}")
+        snap_g = snapshot(g)
+        for name in snap_o:
+            np.testing.assert_array_equal(snap_g[name], snap_o[name], err_msg=f"staged step {k + 1} {name}")
+    g.close()
+
+
+def test_staged_batch_matches_oracle(gpu, oracle_lib):
+    """A batch (one staging per fluid range of every scene) against each scene's oracle."""
+    oracle_lib.set_threads(THREADS)
+    batch = _batch_subset("batch_64")
+    print(f"case staged batch N={[s.fluid_count for s in batch]}", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    g.load_scenes(batch, work_order=STAGED)
+    sb = [g.step_scenes() for _ in range(2)]
+    for i, sc in enumerate(batch):
+        o = oracle_lib.OracleFluid()
+        so = run(o, sc, 2)
+        assert_same_stats([s[i] for s in sb], so, sc.name)
+        g.select_scene(i)
+        assert_same_state(g, o, tag=sc.name)
+    g.close()
