@@ -284,6 +284,14 @@ int pcisph_mlp_get_params(pcisph_solver* s, float* params, size_t count);
 int pcisph_mlp_forward(pcisph_solver* s, const float* x, float* y, size_t n);
 /* Gradient of 0.5 * sum (net(x) - t)^2 / n w.r.t. the parameters, and that loss. */
 int pcisph_mlp_gradients(pcisph_solver* s, const float* x, const float* t, size_t n, float* grad, float* loss);
+/* pcisph_mlp_gradients without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under graph capture, tight serving or
+ * training loops, harnesses that forbid hipMalloc).  Its device scratch (the staged rows
+ * and the accumulators) is a static per-GPU array of ~2.4 MB, so n is at most 65,536
+ * rows (PCISPH_ERR_SIZE beyond); the network's parameters must already be on the device
+ * (pcisph_mlp_set_params).  Calls on one GPU share that array: overlapping calls race. */
+int pcisph_mlp_gradients_noalloc(pcisph_solver* s, const float* x, const float* t, size_t n, float* grad,
+                                 float* loss);
 /* Device sample store: (features, converged pressure / p_ref) pairs. */
 int pcisph_samples_reserve(pcisph_solver* s, size_t capacity);  /* clears the store */
 /* Every following step appends the samples of every `stride`-th fluid particle

@@ -34,6 +34,7 @@ EXPORTS = (
     "pcisph_download_f64", "pcisph_step", "pcisph_run", "pcisph_sync",
     "pcisph_compute_params", "pcisph_profile_enable", "pcisph_profile_read", "pcisph_prepare",
     "pcisph_mlp_set_params", "pcisph_mlp_get_params", "pcisph_mlp_forward", "pcisph_mlp_gradients",
+    "pcisph_mlp_gradients_noalloc",
     "pcisph_samples_reserve", "pcisph_samples_collect", "pcisph_samples_count",
     "pcisph_samples_download", "pcisph_samples_upload", "pcisph_mlp_train", "pcisph_set_warm_start",
     "pcisph_autotune", "pcisph_translate_boundary",

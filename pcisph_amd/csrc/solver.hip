@@ -19,6 +19,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -1618,6 +1619,30 @@ struct pcisph_solver {
 
 static thread_local std::string g_create_error;
 
+// ------------------------------------------------------------------ gradients without allocation
+// The scratch of pcisph_mlp_gradients_noalloc: the caller's rows staged on the device
+// (x, then t) and the gradient and loss accumulators, for at most kNoAllocRows rows.
+// Each GPU holds its own copy of the array until the process ends.  Its device
+// address is taken once per device, at pcisph_create with the device current
+// (hipGetSymbolAddress), and host code uses only that address, never the array's name.
+constexpr size_t kNoAllocRows = 65536;
+constexpr size_t kNoAllocGrad = (kNoAllocRows * (kMlpIn + 1) + 63) / 64 * 64;  // accumulators' offset
+constexpr size_t kNoAllocWords = kNoAllocGrad + (kMlpParams + 1 + 63) / 64 * 64;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocWords];
+static float* g_scratch[kNoAllocDevices];
+static std::mutex g_scratch_mu;
+
+// Once per device, with that device current: the device address of `scratch`.
+static void init_scratch(int device) {
+    if (device < 0 || device >= kNoAllocDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = (float*)p;
+}
+
 template <typename Fn>
 static int guarded(pcisph_solver* s, Fn&& fn) {
     if (!s) return PCISPH_ERR_INVALID_ARGUMENT;
@@ -1668,6 +1693,7 @@ int pcisph_create(int device_ordinal, void* hip_stream, pcisph_solver** out) {
     s->device = device_ordinal;
     int rc = guarded(s, [&] {
         HIP_CHECK(hipSetDevice(device_ordinal));
+        init_scratch(device_ordinal);
         if (hip_stream) {
             s->stream = (hipStream_t)hip_stream;
         } else {
@@ -2041,6 +2067,34 @@ int pcisph_mlp_gradients(pcisph_solver* s, const float* x, const float* t, size_
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(grad, s->mlp_grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s->stream));
         HIP_CHECK(hipMemcpyAsync(loss, s->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+// pcisph_mlp_gradients with its device scratch in the static array above: no device
+// memory is allocated or freed.  Every call on one GPU shares that GPU's array (the
+// accumulators are cleared by kernels at each call); calls that overlap on one GPU
+// race.
+int pcisph_mlp_gradients_noalloc(pcisph_solver* s, const float* x, const float* t, size_t n, float* grad,
+                                 float* loss) {
+    return guarded(s, [&] {
+        if (!x || !t || !grad || !loss || !n) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "null pointer or empty batch");
+        if (n > kNoAllocRows)
+            throw Error(PCISPH_ERR_SIZE, "at most " + std::to_string(kNoAllocRows) + " rows per call");
+        if (!s->mlp) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "no network parameters (pcisph_mlp_set_params first)");
+        float* base = s->device >= 0 && s->device < kNoAllocDevices ? g_scratch[s->device] : nullptr;
+        if (!base) throw Error(PCISPH_ERR_INTERNAL, "no scratch address for this device");
+        HIP_CHECK(hipSetDevice(s->device));
+        float* dx = base;
+        float* dt = base + kNoAllocRows * kMlpIn;
+        float* dg = base + kNoAllocGrad;
+        float* dl = dg + kMlpParams;
+        HIP_CHECK(hipMemcpyAsync(dx, x, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        HIP_CHECK(hipMemcpyAsync(dt, t, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        mlp_gradients(MlpLaunch{s->stream}, s->mlp, dx, dt, (uint32_t)n, dg, dl);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(grad, dg, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, s->stream));
         HIP_CHECK(hipStreamSynchronize(s->stream));
     });
 }
