@@ -41,33 +41,58 @@ STAGES = ["boundary_sort", "hash", "sort", "reorder", "density", "normal", "forc
           "pressure", "pressure_force"]
 
 
-def cpu_baseline(seconds: float, max_steps: int) -> dict:
+def cpu_opencl_platforms() -> list[str]:
+    """OpenCL installable client drivers registered on this host (/etc/OpenCL/vendors);
+    a CPU runtime such as PoCL would be listed here.  Nothing is loaded or run."""
+    d = "/etc/OpenCL/vendors"
+    try:
+        return sorted(os.listdir(d))
+    except OSError:
+        return []
+
+
+def cpu_baseline(steps: int = 200, warmup: int = 5, repeats: int = 3, single_steps: int = 20) -> dict:
     """The oracle (FP32 restatement of the reference step) on the 16,384-particle dam
-    break, from its first step, until `seconds` of CPU work or `max_steps` steps."""
+    break, as BASELINE.md plans it: `warmup` untimed steps from rest, then `steps` timed
+    ones, median of `repeats` runs on the host threads this process may use (OpenMP,
+    OMP_NUM_THREADS, else every core), plus one 1-thread run of `single_steps` steps."""
     from oracle import oracle_py
     from pcisph_amd import scenes
-    threads = min(os.cpu_count() or 1, 16)
-    oracle_py.set_threads(threads)
-    scene = scenes.dam_break_16k()
-    o = oracle_py.OracleFluid()
-    o.load_scene(scene)
-    its = []
-    t0 = time.perf_counter()
-    while len(its) < max_steps and time.perf_counter() - t0 < seconds:
-        its.append(o.step()["iterations"])
-    dt = time.perf_counter() - t0
-    return {"value": scene.fluid_count * len(its) / dt, "unit": "particle-steps/s",
-            "cores": threads, "kind": "port",
-            "sample": f"dam_break_16k (16,384 fluid + {scene.boundary_count} boundary particles) "
-                      f"steps 1..{len(its)} from rest, oracle FP32 C++/OpenMP restatement, "
-                      f"{threads} threads, mean {np.mean(its):.2f} iterations/step"}
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+
+    def run(n_threads, n_steps):
+        oracle_py.set_threads(n_threads)
+        scene = scenes.dam_break_16k()
+        o = oracle_py.OracleFluid()
+        o.load_scene(scene)
+        for _ in range(warmup):
+            o.step()
+        t0 = time.perf_counter()
+        its = [o.step()["iterations"] for _ in range(n_steps)]
+        dt = time.perf_counter() - t0
+        return scene.fluid_count * n_steps / dt, float(np.mean(its)), scene.boundary_count
+
+    multi = [run(threads, steps) for _ in range(repeats)]
+    value = float(np.median([m[0] for m in multi]))
+    single = run(1, single_steps)
+    icds = cpu_opencl_platforms()
+    return {"value": round(value, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "single_thread_value": round(single[0], 1),
+            "cpu_opencl_icds": icds,
+            "sample": f"dam_break_16k (16,384 fluid + {multi[0][2]} boundary particles): {warmup} untimed steps "
+                      f"from rest, then steps {warmup + 1}..{warmup + steps} timed, median of {repeats} runs on "
+                      f"{threads} threads (nproc {nproc}; OMP_NUM_THREADS sets this process's share), mean "
+                      f"{multi[0][1]:.2f} iterations/step; 1 thread: steps {warmup + 1}..{warmup + single_steps}; "
+                      f"oracle FP32 C++/OpenMP restatement of the reference kernels (no CPU OpenCL runtime: "
+                      f"{'ICDs ' + ', '.join(icds) if icds else 'no OpenCL ICD registered'})"}
 
 
 # HBM-side bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
 # the same counters on a copy of known size) for the plain 1M dam break.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_dam_break_1m_plain.json")
 TRAFFIC_KERNELS = {"density": "k_density", "normal": "k_normal", "force_init": "k_force_init",
-                   "pressure": "k_pressure", "pressure_force": "k_pressure_force"}
+                   "pressure": "k_pressure_staged", "pressure_force": "k_pressure_force"}
 
 
 def measured_traffic(stage: str):
@@ -84,8 +109,21 @@ def measured_traffic(stage: str):
     return None
 
 
-def roofline_of(times: dict, n: int, cf: float, cb: float, traffic: bool = False) -> tuple[dict, dict]:
-    """Per-stage device times and the roofline figure of the dominant neighbour kernel."""
+# Compulsory HBM bytes of one launch per fluid particle (and per boundary particle):
+# every array the kernel reads or writes touched once -- own position, predicted
+# position, pressure in; density variation, pressure, x + p/rho^2 out (the pressure
+# pass), the boundary's position, init density and pressure (its boundary mode) -- and
+# the cell table read once.
+COMPULSORY = {"pressure": (16 + 16 + 4 + 4 + 4 + 16, 16 + 4 + 4 + 16)}
+
+
+def roofline_of(times: dict, n: int, cf: float, cb: float, traffic: bool = False, nb: int = 0,
+                buckets: int = 0) -> tuple[dict, dict]:
+    """Per-stage device times and the HBM roofline of the dominant kernel: measured
+    bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy) over its
+    mean HIP-event duration, against 8 TB/s.  Beside it, the kernel's compulsory bytes
+    and the gather model (every candidate's fields read once per evaluating particle,
+    cache hits included) over the same duration."""
     stage_ms = {STAGES[i]: {"ms_total": round(times["ms"][i], 3), "launches": times["launches"][i],
                             "us_per_launch": round(1e3 * times["ms"][i] / times["launches"][i], 2)
                             if times["launches"][i] else None}
@@ -93,17 +131,25 @@ def roofline_of(times: dict, n: int, cf: float, cb: float, traffic: bool = False
     dom = max(STAGE_BYTES, key=lambda k: times["ms"][STAGES.index(k)])
     i = STAGES.index(dom)
     avg_s = times["ms"][i] / times["launches"][i] / 1e3
-    bytes_per_launch = STAGE_BYTES[dom](cf, cb) * n
-    achieved = bytes_per_launch / avg_s / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(dom) if traffic else None,
-                "bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
+    gather = STAGE_BYTES[dom](cf, cb) * n
+    measured = measured_traffic(dom) if traffic else None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": measured, "avg_launch_us": round(avg_s * 1e6, 2),
+                "bytes_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch (" +
+                                os.path.relpath(TRAFFIC_FILE, ROOT) + ") / HIP-event mean duration",
+                "gather_model_bytes": int(gather), "gather_model_GBs": round(gather / avg_s / 1e9, 1)}
+    if dom in COMPULSORY:
+        per_f, per_b = COMPULSORY[dom]
+        comp = per_f * n + per_b * nb + 8 * buckets
+        roofline.update({"compulsory_bytes": int(comp), "compulsory_GBs": round(comp / avg_s / 1e9, 1)})
+    if measured:
+        achieved = measured / avg_s / 1e9
+        roofline.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4)})
     return stage_ms, roofline
 
 
 def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, traffic: bool = False,
-            scene=None, profile_every: int = 8) -> dict:
+            scene=None, profile_every: int = 8, settle: int = 0) -> dict:
     """`warmup` untimed then `steps` timed steps (every scene of a batch), HIP-event
     stage times over the timed region when `profile`: every `profile_every`-th step of
     it is launched kernel by kernel between events, the others replay their HIP graphs
@@ -128,13 +174,13 @@ def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, t
             lat.append(time.perf_counter() - t)
         return out
 
-    go(warmup, 0)
+    go(settle + warmup, 0)
     fluid.sync()
     lat.clear()
     if profile:
         fluid.profile_enable(True, every=profile_every)
     t0 = time.perf_counter()
-    stats = go(steps, warmup)
+    stats = go(steps, settle + warmup)
     fluid.sync()
     elapsed = time.perf_counter() - t0
     times = fluid.profile_read() if profile else None
@@ -150,13 +196,18 @@ def measure(fluid, steps: int, warmup: int, n: int, profile: bool, batch=None, t
            "steps_per_s": round(steps / elapsed, 2), "mean_iterations": round(float(its[live].mean()), 3),
            "max_iterations_seen": int(its.max()), "converged_fraction": round(float(conv[live].mean()), 4),
            "max_density_error_max": round(float(err.max()), 5),
+           "meets_bound": bool(conv[live].all() and err.max() <= 0.01),
            "mean_fluid_candidates": round(cf, 2), "mean_boundary_candidates": round(cb, 2)}
     if batch is not None:
         out["mean_iterations_slowest_scene"] = round(float(its.max(axis=1).mean()), 3)
         # every scene of a batch completes its step when the batch's step returns
         out["step_latency_ms_p50_p99"] = [round(1e3 * float(np.percentile(lat, q)), 3) for q in (50, 99)]
     if times is not None:
-        out["stage_times"], out["roofline"] = roofline_of(times, n, cf, cb, traffic)
+        nb = sum(sc.boundary_count for sc in batch) if batch is not None else (scene.boundary_count if scene else 0)
+        buckets = fluid.get_params().bucket_count if batch is None else 0
+        out["stage_times"], out["roofline"] = roofline_of(times, n, cf, cb, traffic, nb, buckets)
+    if settle:
+        out["settled_after_steps"] = settle
     return out
 
 
@@ -206,14 +257,18 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dam_break_1m")
     ap.add_argument("--precision", default="fast", choices=["fast", "ieee", "fp64"])
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-steps", type=int, default=200, help="timed oracle steps per CPU-baseline run")
+    ap.add_argument("--settle-steps", type=int, default=400,
+                    help="untimed steps before the timed ones of the plain_settled form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
     ap.add_argument("--profile-every", type=int, default=8,
                     help="time every n-th step of the timed region kernel by kernel between HIP events")
-    ap.add_argument("--forms", default="plain,warm_start,batch_64,batch_256,batch_256_warm,sloshing_plain,"
-                                       "sloshing_warm,gain,gain_sloshing",
-                    help="comma list of plain (reference loop), previous_pressure (each step starts from "
+    ap.add_argument("--forms", default="plain,plain_settled,warm_start,batch_64,batch_256,batch_256_warm,"
+                                       "sloshing_plain,sloshing_warm,gain,gain_sloshing",
+                    help="comma list of plain (reference loop), plain_settled (the reference loop after "
+                         "--settle-steps untimed steps from rest, once the start-up transient has passed), "
+                         "previous_pressure (each step starts from "
                          "the previous pressures), warm_start (trained network), batch_64 (64-scene batch), "
                          "sloshing_plain / sloshing_warm (the 1M block in a tank oscillating along x, "
                          "0.5 m at 0.8 Hz; plain loop / trained network), gain / gain_sloshing (iterations "
@@ -268,6 +323,9 @@ def main() -> int:
             total = sum(sc.fluid_count for sc in batch)
             fluid = pcisph_amd.Fluid(device=0, precision=prec)
             fluid.load_scenes(batch)
+            tuned = None
+            if form == "batch_64":  # block size and radix digit width from the autotuner
+                tuned = fluid.autotune(trial_steps=2)
             if form == "batch_256_warm":
                 if not os.path.exists(args.weights):
                     continue
@@ -282,6 +340,8 @@ def main() -> int:
             r.pop("roofline", None)
             r.update({"scenes": len(batch), "fluid_particles": total,
                       "boundary_particles": sum(sc.boundary_count for sc in batch)})
+            if tuned:
+                r["autotuned"] = tuned
         else:
             sc = scenes.sloshing_tank(0.5, 0.8, getattr(scenes, args.workload)()) \
                 if form.startswith("sloshing") else scene
@@ -299,8 +359,9 @@ def main() -> int:
             print(f"bench: {form} {sc.name} N={n} Nb={sc.boundary_count} warmup={args.warmup} "
                   f"steps={args.steps}", file=sys.stderr, flush=True)
             r = measure(fluid, args.steps, args.warmup, n, not args.no_profile,
-                        traffic=form == "plain" and args.workload == "dam_break_1m", scene=sc,
-                        profile_every=args.profile_every)
+                        traffic=form in ("plain", "plain_settled") and args.workload == "dam_break_1m", scene=sc,
+                        profile_every=args.profile_every,
+                        settle=args.settle_steps if form == "plain_settled" else 0)
         fluid.close()
         results[form] = r
         print(f"bench: {form} {json.dumps({k: v for k, v in r.items() if k != 'stage_times'})}",
@@ -316,7 +377,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_seconds, 200)
+        cpu = cpu_baseline(steps=args.cpu_steps)
 
     line = {
         "metric": METRIC, "value": main_r["value"], "unit": "particle-steps/s", "n_gpus": 1,

@@ -42,18 +42,9 @@ void SphLaunch<R, FAST>::pressure(const Launch& L, const DeviceArrays<R>& A, uin
     if (!grid) return;
     if constexpr (std::is_same<R, float>::value) {
         if (A.staged_pressure) {
-            // fluid ranges from the step's LDS staging, boundary ranges as below
-            if (B.fluid_blocks)
-                hipLaunchKernelGGL((k_pressure_staged<FAST>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B,
-                                   iter, A.bcells, A.bpos, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.stage, A.NL);
-            if (B.boundary_blocks) {
-                Batch<R> Bb = B;
-                Bb.blocks = B.blocks + B.fluid_blocks;
-                Bb.fluid_blocks = 0;
-                hipLaunchKernelGGL((k_pressure<R, FAST>), dim3(B.boundary_blocks), dim3(L.block), 0, L.stream, Bb,
-                                   iter, A.bcells, A.bpos, A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp,
-                                   A.bxf, A.NL);
-            }
+            // fluid ranges from the step's LDS staging, boundary ranges in the same launch
+            hipLaunchKernelGGL((k_pressure_staged<FAST>), dim3(grid), dim3(L.block), 0, L.stream, B, iter, A.bcells,
+                               A.bpos, A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp, A.bxf, A.stage, A.NL);
             return;
         }
         if (A.tile_pressure && L.block == (uint32_t)kTileThreads) {

@@ -261,14 +261,17 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
     if (threadIdx.x == 0) S.info[tile] = total;
 }
 
-// update_pressure (pcisph.cl:212-263) on the fluid ranges, candidates from the
-// workgroup's LDS staging (see above); the same results as k_pressure's bucket walk.
-// Fluid ranges only; `blocks` entries carry their staging index (kRangeTileShift).
+// update_pressure (pcisph.cl:212-263), the fluid ranges with their candidates from the
+// workgroup's LDS staging (see above) -- the same results as k_pressure's bucket walk --
+// and the boundary ranges (boundary mode) walking their fluid buckets in global memory,
+// in one launch so both kinds of workgroups share the device.  Fluid `blocks` entries
+// carry their staging index (kRangeTileShift).
 template <bool FAST>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
     Batch<float> B, uint32_t iter, const uint2* __restrict__ bcells, const float4* __restrict__ bpos,
-    const uint2* __restrict__ fcells, const float4* __restrict__ xr, const float4* __restrict__ xp,
-    float* __restrict__ dv, float* __restrict__ p, float4* __restrict__ xf, StageMaps S, NeighborLists NL) {
+    const float* __restrict__ binit, const uint2* __restrict__ fcells, const float4* __restrict__ xr,
+    const float4* __restrict__ xp, float* __restrict__ dv, float* __restrict__ p, float4* __restrict__ xf,
+    float* __restrict__ bp, float4* __restrict__ bxf, StageMaps S, NeighborLists NL) {
     using O = Ops<float, FAST>;
     __shared__ float s_x[kStageCap], s_y[kStageCap], s_z[kStageCap];
     __shared__ unsigned long long lds[2][8];
@@ -277,6 +280,32 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
     const SphParams<float> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
     if (iteration_skipped<float>(st, iter, P)) return;
+    if (rg.flags & kRangeBoundary) {
+        // boundary mode (pcisph.cl:231-236): binit + the fluid candidates at x^
+        const uint32_t b = rg.begin + threadIdx.x;
+        if (b < rg.end) {
+            const float4 self = bpos[b];
+            const int3 c = cell_of4<float>(self, P.cell);
+            float pd = 0.f;
+            pd += binit[b];
+            walk_buckets<float>(cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xp[j]; },
+                                [&](const float4& o, uint32_t) {
+                                    float dx, dy, dz;
+                                    pd += poly6<float>(dist2<float>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz),
+                                                       P.h2);
+                                });
+            pd *= P.density_scale;
+            const float v = fmaxf(0.f, pd - P.rho0);
+            if (v != 0.f) {
+                const float pb = bp[b] + v * P.delta;
+                bp[b] = pb;
+                float4 out = self;
+                out.w = O::div(pb, P.rho0_sq);
+                bxf[b] = out;
+            }
+        }
+        return;  // boundary ranges take no part in the fluid maxima
+    }
     const uint32_t tile = rg.flags >> kRangeTileShift;
     const uint32_t total = S.info[tile];
     const bool staged = total != kStageFallback;

@@ -2,19 +2,29 @@
 // (PCISPH/src/main.cpp:22-258) without a window.  OpenGL rendering and PNG recording
 // are replaced by particle-frame dumps, which double as checkpoints.
 //
-//   pcisph_headless -i <scene> [-n steps] [-o outdir] [-e every] [-d ms] [--resume frame]
-//                   [--precision fast|ieee|fp64] [--max-iterations k] [--emit-scene file]
+//   pcisph_headless -i <scene> [-o] [-d ms] [-n steps] [--out-dir dir] [--final file]
+//                   [--resume frame] [--precision fast|ieee|fp64] [--max-iterations k]
+//                   [--emit-scene file]
 //
-// <scene>: dam_break_16k | dam_break_1m | xraw:<dir>:<dambreak|cube_splash|simple_drop>
-// -o writes frame_<step>.pcsf every `-e` steps (reference: -o records PNG frames,
-// main.cpp:36-54); -d sleeps that many ms per step (reference: frame delay).
+// The reference's arguments keep their meaning (main.cpp:36-54):
+//   -i <scene>  the scene: dam_break_16k | dam_break_1m | xraw:<dir>:<dambreak|cube_splash|simple_drop>
+//   -o          record: a frame whenever at least 1/60 s of simulated time has passed since
+//               the last one (the first after the first step), <out-dir>/frame_<k>.pcsf for
+//               k = 0, 1, ... (main.cpp:193-222, where frames are out/frame_<k>.png)
+//   -d <ms>     stop once the simulated time reaches ms / 1000 s (main.cpp:43-45,224)
+// Additions: -n caps the step count (the scene's own step count when neither -n nor -d
+// is given: there is no window to close); --out-dir (default "out"); --final writes the
+// last state as a frame; --resume continues from a frame (its step count and simulated
+// time); --emit-scene writes the generated scene without stepping it.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
+#include <limits>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/pcisph.h"
@@ -82,9 +92,11 @@ bool read_frame(const std::string& path, scene_gen::Scene& sc, std::vector<float
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::string scene_name = "dam_break_16k", outdir, resume, emit, precision = "fast";
-    uint32_t steps = 0, every = 10, max_it = 0;
-    int delay_ms = 0;
+    std::string scene_name, outdir = "out", resume, emit, precision = "fast", final_path;
+    uint32_t steps = 0, max_it = 0;
+    bool recording = false;
+    float duration = std::numeric_limits<float>::infinity();
+    const float interval = 1.f / 60.f;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -94,20 +106,29 @@ int main(int argc, char** argv) {
             }
             return argv[++i];
         };
-        if (a == "-i") scene_name = next();
-        else if (a == "-n") steps = (uint32_t)std::stoul(next());
-        else if (a == "-o") outdir = next();
-        else if (a == "-e") every = (uint32_t)std::stoul(next());
-        else if (a == "-d") delay_ms = std::stoi(next());
-        else if (a == "--resume") resume = next();
-        else if (a == "--precision") precision = next();
-        else if (a == "--max-iterations") max_it = (uint32_t)std::stoul(next());
-        else if (a == "--emit-scene") emit = next();
-        else {
-            std::fprintf(stderr, "usage: %s -i <scene> [-n steps] [-o dir] [-e every] [-d ms] [--resume frame] "
-                                 "[--precision fast|ieee|fp64] [--max-iterations k] [--emit-scene file]\n", argv[0]);
+        try {
+            if (a == "-i") scene_name = next();
+            else if (a == "-o") recording = true;
+            else if (a == "-d") duration = 0.001f * (float)std::stoi(next());
+            else if (a == "-n") steps = (uint32_t)std::stoul(next());
+            else if (a == "--out-dir") outdir = next();
+            else if (a == "--final") final_path = next();
+            else if (a == "--resume") resume = next();
+            else if (a == "--precision") precision = next();
+            else if (a == "--max-iterations") max_it = (uint32_t)std::stoul(next());
+            else if (a == "--emit-scene") emit = next();
+            else {
+                std::printf("Invalid argument: %s\n", a.c_str());
+                return 2;
+            }
+        } catch (const std::exception&) {
+            std::printf("Invalid value for %s\n", a.c_str());
             return 2;
         }
+    }
+    if (scene_name.empty()) {
+        std::printf("-i <scene_name>\n");
+        return 2;
     }
     scene_gen::Scene sc;
     try {
@@ -132,7 +153,7 @@ int main(int argc, char** argv) {
         std::printf("scene %s: %u fluid, %u boundary particles\n", sc.name.c_str(), n, nb);
         return f ? 0 : 1;
     }
-    if (!steps) steps = sc.steps;
+    if (!steps && !std::isfinite(duration)) steps = sc.steps;
     std::vector<float> vel;
     uint64_t step0 = 0;
     if (!resume.empty() && !read_frame(resume, sc, vel, step0)) {
@@ -172,31 +193,50 @@ int main(int argc, char** argv) {
         check(s, pcisph_prepare(s), "pcisph_prepare");
         check(s, pcisph_upload(s, PCISPH_FLUID_VELOCITIES, vel.data(), vel.size()), "upload velocities");
     }
-    std::printf("scene %s: %zu fluid, %zu boundary particles, %u steps\n", sc.name.c_str(), sc.fluid.size() / 3,
-                sc.boundary.size() / 3, steps);
-    double it_sum = 0, err_max = 0;
-    const auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t k = 0; k < steps; ++k) {
-        pcisph_step_stats st;
-        check(s, pcisph_step(s, &st), "pcisph_step");
-        it_sum += st.iterations;
-        err_max = std::max(err_max, (double)st.max_density_error);
-        if (st.nonfinite) {
-            std::fprintf(stderr, "non-finite state at step %llu\n", (unsigned long long)(step0 + k + 1));
-            return 3;
+    std::printf("scene %s: %zu fluid, %zu boundary particles\n", sc.name.c_str(), sc.fluid.size() / 3,
+                sc.boundary.size() / 3);
+    // main.cpp:157-225: step, advance the simulated time; while recording, step until a
+    // frame interval has passed, then record; stop at the duration
+    if (recording) {
+        std::error_code ec;
+        std::filesystem::create_directories(outdir, ec);
+        if (ec) {
+            std::fprintf(stderr, "cannot create %s: %s\n", outdir.c_str(), ec.message().c_str());
+            return 1;
         }
-        const uint64_t step = step0 + k + 1;
-        if (!outdir.empty() && (step % every == 0 || k + 1 == steps)) {
-            char name[64];
-            std::snprintf(name, sizeof(name), "/frame_%06llu.pcsf", (unsigned long long)step);
-            write_frame(s, outdir + name, step);
-        }
-        if (delay_ms) std::this_thread::sleep_for(std::chrono::milliseconds(delay_ms));
     }
+    float sim_time = (float)step0 * sc.delta_t;
+    float last_frame = -std::numeric_limits<float>::infinity();
+    uint32_t frame = 0, done = 0;
+    double it_sum = 0, err_max = 0;
+    auto limit = [&] { return (steps && done >= steps) || sim_time >= duration; };
+    const auto t0 = std::chrono::steady_clock::now();
+    while (!limit()) {
+        do {
+            pcisph_step_stats st;
+            check(s, pcisph_step(s, &st), "pcisph_step");
+            ++done;
+            sim_time += sc.delta_t;
+            it_sum += st.iterations;
+            err_max = std::max(err_max, (double)st.max_density_error);
+            if (st.nonfinite) {
+                std::fprintf(stderr, "non-finite state at step %llu\n", (unsigned long long)(step0 + done));
+                return 3;
+            }
+        } while (recording && sim_time - last_frame <= interval && !limit());
+        if (recording && sim_time - last_frame >= interval) {
+            write_frame(s, outdir + "/frame_" + std::to_string(frame) + ".pcsf", step0 + done);
+            ++frame;
+            last_frame = sim_time;
+        }
+    }
+    if (!final_path.empty()) write_frame(s, final_path, step0 + done);
     check(s, pcisph_sync(s), "pcisph_sync");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    std::printf("%u steps in %.3f s: %.1f steps/s, %.4g particle-steps/s, mean %.2f iterations, max error %.4f\n",
-                steps, secs, steps / secs, (sc.fluid.size() / 3) * steps / secs, it_sum / steps, err_max);
+    std::printf("%u steps (simulated time %.4f s, %u frames) in %.3f s: %.1f steps/s, %.4g particle-steps/s, "
+                "mean %.2f iterations, max error %.4f\n",
+                done, sim_time, frame, secs, done / secs, (sc.fluid.size() / 3) * done / secs,
+                done ? it_sum / done : 0.0, err_max);
     pcisph_destroy(s);
     return 0;
 }
