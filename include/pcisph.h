@@ -148,7 +148,9 @@ typedef struct pcisph_step_stats {
     float max_density_error;        /* max_density_variation / rest_density (FP32, as tested) */
     uint32_t nonfinite;             /* 1 if a non-finite position/velocity was produced */
     uint32_t warm_start;            /* bit 0: the network's start was evaluated; bit 1: it was
-                                       rejected for the plain start (larger residual) */
+                                       rejected for the plain start (larger residual); bit 2:
+                                       the scene is left out of the next step's warm start
+                                       (see pcisph_set_warm_start) */
     uint64_t fluid_candidates;      /* sum over fluid particles of fluid candidates */
     uint64_t boundary_candidates;   /* sum over fluid particles of boundary candidates */
     float max_density_expansion;    /* max of max(0, rho0 - rho*) over fluid particles whose
@@ -158,6 +160,10 @@ typedef struct pcisph_step_stats {
     uint32_t list_waves;            /* fluid wavefronts (64 particles) of this step's pressure
                                        passes that walked neighbour lists; the others walked
                                        their 27 buckets (sum over the step's iterations) */
+    float first_density_error;      /* max density error of the plain (zero-pressure) start
+                                       after the first correction iteration; in a warm-started
+                                       step the dual first iteration evaluates it beside the
+                                       network's start */
 } pcisph_step_stats;
 
 typedef struct pcisph_solver pcisph_solver;
@@ -303,6 +309,15 @@ int pcisph_samples_upload(pcisph_solver* s, const float* x, const float* t, size
 /* `steps` Adam steps on random batches of the store; mean loss of the last step. */
 int pcisph_mlp_train(pcisph_solver* s, uint32_t steps, uint32_t batch, float learning_rate, uint64_t seed,
                      float* last_loss);
+/* Steps start the correction loop from the network (max(0, p_prev + y p_ref) per
+ * particle; the plain zero-pressure start is evaluated beside it in the first
+ * iteration and kept where its residual is smaller).  A scene is left out of a
+ * step's warm start, and runs exactly the plain step, after a step that ran to the
+ * iteration cap without converging, or that converged at the minimum iteration count
+ * with the plain start (or with the network's start while the plain start's first
+ * iteration was already within the bound): no start shortens such a loop.  It is
+ * taken back after a step that needed more than the minimum.  Enabling clears every
+ * scene's left-out mark. */
 int pcisph_set_warm_start(pcisph_solver* s, int enable);
 
 /* ---- batches of scenes (no counterpart in the reference) -------------------------

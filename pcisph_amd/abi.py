@@ -70,6 +70,7 @@ class StepStats(C.Structure):
         ("boundary_candidates", C.c_uint64),
         ("max_density_expansion", C.c_float),
         ("list_waves", C.c_uint32),
+        ("first_density_error", C.c_float),
     ]
 
     def as_dict(self) -> dict:

@@ -34,9 +34,14 @@ void mlp_adam(const MlpLaunch& L, float* params, const float* grad, float* m, fl
               float beta1, float beta2, float eps);
 
 // Warm start: y[i] = net(feat[i]), the change of the particle's pressure from the
-// previous step's converged value, in units of the scene's p_ref.
+// previous step's converged value, in units of the scene's p_ref, for the rows i of
+// the ranges in `rows` (device memory: {count, tiles, then per range {begin, end,
+// first tile}}, ranges non-empty and increasing, first tile = the sum of
+// ceil((end - begin) / 16) over the ranges before it, tiles = that sum over all).
+// The launch is sized for `max_rows` rows, at least the ranges' total.
 template <typename R>
-void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, uint32_t n, R* y);
+void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, const uint32_t* rows,
+                    uint32_t max_rows, R* y);
 // Appends (feat[i], p[i] / p_ref - feat[i][2]) -- the converged pressure's change from
 // the previous step's, whose p / p_ref is input 2 -- for i = phase, phase + stride, ...
 // to the sample store at slots base + i / stride (those below cap).

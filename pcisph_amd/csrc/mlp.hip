@@ -316,20 +316,44 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, floa
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// Inference over the row ranges of `rows` (layout in mlp.h): each wavefront takes a
+// contiguous run of 16-row tiles, finds the range of its first tile by bisection and
+// walks on from there; rows past a range's end are zero inputs and not stored.
+// Workgroups with no tile return before staging the weights.
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_mlp_warm(const float* __restrict__ P, const float* __restrict__ x,
-                                                       uint32_t n, R* __restrict__ y) {
+                                                       const uint32_t* __restrict__ rows, R* __restrict__ y) {
     __shared__ Weights s;
+    const uint32_t count = rows[0], tiles = rows[1];
+    const uint32_t waves = gridDim.x * kWaves;
+    const uint32_t per = (tiles + waves - 1) / waves;
+    if ((uint64_t)blockIdx.x * kWaves * per >= tiles) return;  // whole workgroup idle
     stage_weights(s, P);
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
-    const uint32_t waves = gridDim.x * kWaves;
-    for (uint32_t row0 = wave * 16; row0 < n; row0 += waves * 16) {
-        const uint32_t row = row0 + (lane & 15);
+    uint32_t q = wave * per;
+    const uint32_t qe = q + per < tiles ? q + per : tiles;
+    if (q >= qe) return;
+    const uint32_t* r = rows + 2;  // {begin, end, first tile} per range, first tiles increasing
+    uint32_t lo = 0, hi = count;   // r[3 lo + 2] <= q < r[3 hi + 2] (hi == count: past the last)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (r[3 * mid + 2] <= q) lo = mid;
+        else hi = mid;
+    }
+    uint32_t k = lo, b = r[3 * k], e = r[3 * k + 1], t0 = r[3 * k + 2];
+    for (; q < qe; ++q) {
+        while (q >= t0 + (e - b + 15) / 16) {
+            ++k;
+            b = r[3 * k];
+            e = r[3 * k + 1];
+            t0 = r[3 * k + 2];
+        }
+        const uint32_t row = b + 16 * (q - t0) + (lane & 15);
         v4f z1[4], z2[4];
-        const float out = tile_forward(s, load_x(x, row, n, g), z1, z2);
-        if (g == 0 && row < n) y[row] = (R)out;
+        const float out = tile_forward(s, load_x(x, row, e, g), z1, z2);
+        if (g == 0 && row < e) y[row] = (R)out;
     }
 }
 
@@ -390,12 +414,15 @@ void mlp_gradients(const MlpLaunch& L, const float* params, const float* x, cons
 }
 
 template <typename R>
-void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, uint32_t n, R* y) {
-    if (!n) return;
-    hipLaunchKernelGGL((k_mlp_warm<R>), dim3(blocks_for(n)), dim3(kThreads), 0, L.stream, params, feat, n, y);
+void mlp_warm_start(const MlpLaunch& L, const float* params, const float* feat, const uint32_t* rows,
+                    uint32_t max_rows, R* y) {
+    if (!max_rows) return;
+    hipLaunchKernelGGL((k_mlp_warm<R>), dim3(blocks_for(max_rows)), dim3(kThreads), 0, L.stream, params, feat, rows,
+                       y);
 }
-template void mlp_warm_start<float>(const MlpLaunch&, const float*, const float*, uint32_t, float*);
-template void mlp_warm_start<double>(const MlpLaunch&, const float*, const float*, uint32_t, double*);
+template void mlp_warm_start<float>(const MlpLaunch&, const float*, const float*, const uint32_t*, uint32_t, float*);
+template void mlp_warm_start<double>(const MlpLaunch&, const float*, const float*, const uint32_t*, uint32_t,
+                                     double*);
 
 template <typename R>
 void mlp_collect(const MlpLaunch& L, const float* feat, const R* p, uint32_t n, uint32_t stride, uint32_t phase,

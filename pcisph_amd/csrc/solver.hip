@@ -246,6 +246,7 @@ struct StepResult {
     uint64_t fluid_candidates = 0, boundary_candidates = 0;
     bool warm = false, warm_rejected = false;
     uint32_t list_waves = 0;
+    double first_dv = 0;  // the plain start's max density variation after iteration 0
 };
 
 // What a step does for the warm-start network: use it (params != null) and/or
@@ -264,7 +265,7 @@ struct WarmArgs {
 struct SceneInput {
     const Simulation_Params* params;
     float threshold;
-    bool warm = true;  // the warm start applies to this scene (when the step uses it)
+    bool warm = false;  // the step starts this scene from the network
 };
 
 class EngineBase {
@@ -363,6 +364,7 @@ class Engine final : public EngineBase {
         free_internal();
         dfree(staging_);
         dfree(feat_);
+        dfree(warm_rows_);
         drop_snapshot();
         dfree(A_.NL.f);
         dfree(A_.NL.b);
@@ -992,11 +994,13 @@ class Engine final : public EngineBase {
     // Every value a recorded launch bakes in; graphs are re-captured when it changes.
     std::vector<uint64_t> launch_key(const Launch& L, bool fast, bool warm, bool features, uint32_t digit,
                                      uint32_t key_bits, uint32_t maxit, uint32_t chunk, uint32_t layout_bits,
-                                     uint32_t morton_bits, const float* mlp, bool staged) const {
+                                     uint32_t morton_bits, const float* mlp, bool staged,
+                                     bool all_features) const {
         std::vector<uint64_t> k;
         auto put = [&](uint64_t v) { k.push_back(v); };
         auto ptr = [&](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
-        put(L.block), put(fast), put(warm), put(features), put(digit), put(key_bits), put(maxit), put(chunk);
+        put(L.block), put(fast), put(warm), put(features), put(all_features), put(digit), put(key_bits), put(maxit),
+            put(chunk);
         put(staged), ptr(A_.stage.work), ptr(A_.stage.rows), ptr(A_.stage.src), ptr(A_.stage.info);
         put(layout_bits), put(morton_bits);
         put(A_.tile_pressure);
@@ -1008,7 +1012,8 @@ class Engine final : public EngineBase {
                               (const void*)A_.keys[1], (const void*)A_.vals[0], (const void*)A_.vals[1],
                               (const void*)A_.cells, (const void*)A_.bpos, (const void*)A_.bxf, (const void*)A_.bp,
                               (const void*)A_.binit, (const void*)A_.bcells, (const void*)sort_scratch_,
-                              (const void*)feat_, (const void*)mlp, (const void*)ctl_, (const void*)dev_host_,
+                              (const void*)feat_, (const void*)warm_rows_, (const void*)mlp, (const void*)ctl_,
+                              (const void*)dev_host_,
                               (const void*)dev_results_, (const void*)D_.xp0, (const void*)D_.xf0,
                               (const void*)D_.p0, (const void*)D_.dv0, (const void*)D_.bp0, (const void*)D_.bxf0})
             ptr(p);
@@ -1073,6 +1078,7 @@ class Engine final : public EngineBase {
                                  stage_info_};
         }
         if (features) ensure_warm_buffers(warm);
+        if (warm) upload_warm_rows(scenes);
         D_.disp_next = A_.NL.on ? A_.NL.disp[1] : nullptr;
         D_.buckets = A_.NL.buckets;
         if (!ctl_) dalloc(ctl_, 64, stream_);
@@ -1081,7 +1087,8 @@ class Engine final : public EngineBase {
         const uint32_t chunk = prof_.begin_step() ? 0u : o.graph_iterations;
         if (chunk) {
             std::vector<uint64_t> key =
-                launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params, staged);
+                launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params, staged,
+                           w.collect_stride != 0);
             if (key != graph_key_) {
                 drop_graphs();
                 graph_key_ = key;
@@ -1156,13 +1163,13 @@ class Engine final : public EngineBase {
             });
             prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
             prof_.timed(PCISPH_STAGE_NORMAL, stream_, -1, [&] { ops.normal(L, A_); });
-            FeatureArgs F{features ? feat_ : nullptr, A_.counts};
+            FeatureArgs F{features ? feat_ : nullptr, A_.counts, w.collect_stride ? 1u : 0u};
             prof_.timed(PCISPH_STAGE_FORCE_INIT, stream_, -1, [&] { ops.force_init(L, A_, F); });
             if (warm) {
                 // network start: p from the features, its pressure force, and a prediction
                 // with it; the plain prediction is kept for the dual first iteration
                 prof_.timed(PCISPH_STAGE_WARM_START, stream_, -1, [&] {
-                    mlp_warm_start<R>(ML, w.params, feat_, A_.n, A_.p);
+                    mlp_warm_start<R>(ML, w.params, feat_, warm_rows_, A_.n, A_.p);
                     DataLaunch<R>::warm_apply(L, A_);
                     ops.pressure_force(L, A_, 0, dev_host_, dev_results_, ctl_, 1, D_.xp0);
                 });
@@ -1196,8 +1203,10 @@ class Engine final : public EngineBase {
         auto iteration = [&](uint32_t it) {
             const DeviceArrays<R>& A = *AI;
             prof_.timed(PCISPH_STAGE_PRESSURE, stream_, (int)it, [&] {
+                // iteration 0 of a warm-started step: the dual pass for the scenes started
+                // from the network, the plain pass for the others
                 if (warm && it == 0) ops.pressure_dual(L, A, D_);
-                else ops.pressure(L, A, it);
+                ops.pressure(L, A, it);
             });
             prof_.timed(PCISPH_STAGE_PRESSURE_FORCE, stream_, (int)it, [&] {
                 ops.pressure_force(L, A, it, dev_host_, dev_results_, ctl_, 0, nullptr);
@@ -1284,6 +1293,7 @@ class Engine final : public EngineBase {
             r.boundary_candidates = q.candidates[1];
             r.warm = warm && scenes[s].warm;
             r.warm_rejected = r.warm && q.warm_rejected;
+            r.first_dv = (double)Bits<R>::to(q.first_dv);
             r.list_waves = q.list_waves;
         }
         update_list_radius(out);
@@ -1433,6 +1443,38 @@ class Engine final : public EngineBase {
         }
     }
 
+    // The fluid rows of the scenes the step starts from the network, as the inference
+    // kernel reads them (mlp.h): adjacent scenes merged into one range; uploaded when
+    // they differ from the last step's.
+    void upload_warm_rows(const std::vector<SceneInput>& scenes) {
+        std::vector<uint32_t> t{0, 0};
+        for (size_t s = 0; s < scenes.size() && s < n_.size(); ++s) {
+            if (!scenes[s].warm || !n_[s]) continue;
+            const uint32_t b = f_off_[s], e = b + n_[s];
+            if (t.size() > 2 && t[t.size() - 2] == b) {
+                t[t.size() - 2] = e;
+            } else {
+                t.insert(t.end(), {b, e, 0u});
+                ++t[0];
+            }
+        }
+        uint32_t tiles = 0;
+        for (uint32_t k = 0; k < t[0]; ++k) {
+            t[2 + 3 * k + 2] = tiles;
+            tiles += (t[2 + 3 * k + 1] - t[2 + 3 * k] + 15) / 16;
+        }
+        t[1] = tiles;
+        if (t == warm_rows_host_) return;
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        if (t.size() > warm_rows_cap_) {
+            dalloc(warm_rows_, t.size(), stream_);
+            warm_rows_cap_ = t.size();
+        }
+        HIP_CHECK(hipMemcpyAsync(warm_rows_, t.data(), sizeof(uint32_t) * t.size(), hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        warm_rows_host_ = t;
+    }
+
    private:
     struct LaunchTable {
         void (*density)(const Launch&, const DeviceArrays<R>&);
@@ -1479,6 +1521,9 @@ class Engine final : public EngineBase {
     float* staging_ = nullptr;
     size_t staging_words_ = 0;
     float* feat_ = nullptr;
+    uint32_t* warm_rows_ = nullptr;  // the inference kernel's row ranges
+    size_t warm_rows_cap_ = 0;
+    std::vector<uint32_t> warm_rows_host_;
     uint32_t feat_n_ = 0, dual_n_ = 0, dual_nb_ = 0, list_n_ = 0, list_nb_ = 0;
     float list_skin_ = 0.f;
     std::vector<uint8_t> near_;          // per scene: lists of radius (1 + kNearSkin) h this step
@@ -1558,7 +1603,7 @@ struct pcisph_solver {
         bool params_changed = true;
         bool boundary_updated = true;
         float threshold = 0.01f;
-        bool capped = false;  // its last step ran to the iteration cap without converging
+        bool warm_skip = false;  // left out of the next step's warm start (warm_start_eligible)
     };
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1860,6 +1905,23 @@ static void apply_param_changes(pcisph_solver* s) {
     }
 }
 
+// Whether the next step may start a scene from the network, from its last step.  Not
+// after a step that ran to the iteration cap without converging: a start cannot
+// shorten a loop the cap ends.  Nor after a step that converged at the minimum
+// iteration count unless the network's start was kept there and the plain start's
+// first iteration was above the bound: no start can go below the minimum, and the
+// warm start then costs its inference, network pressure force and dual first
+// iteration for nothing.  A scene left out runs plain steps until one needs more
+// than the minimum.
+static bool warm_start_eligible(const StepResult& r, const pcisph_solver_options& o, double rest_density,
+                                double threshold) {
+    const uint32_t floor_it = std::max(o.min_iterations, 1u);
+    if (r.iterations >= o.max_iterations && !r.converged) return false;
+    if (!(r.converged && r.iterations <= floor_it)) return true;
+    const bool kept = r.warm && !r.warm_rejected;
+    return kept && !(r.first_dv / rest_density < threshold);
+}
+
 static pcisph_step_stats to_stats(const pcisph_solver* s, const StepResult& r, const Simulation_Params& p) {
     pcisph_step_stats st{};
     if (!p.fluid_count) return st;
@@ -1873,6 +1935,7 @@ static pcisph_step_stats to_stats(const pcisph_solver* s, const StepResult& r, c
     if (s->precision != kPrecisionFp64) st.max_density_error = (float)r.max_dv / p.rest_density;
     st.fluid_candidates = r.fluid_candidates;
     st.boundary_candidates = r.boundary_candidates;
+    st.first_density_error = (float)(r.first_dv / (double)p.rest_density);
     return st;
 }
 
@@ -1896,6 +1959,7 @@ static pcisph_step_stats combine(const std::vector<pcisph_step_stats>& v) {
         a.fluid_candidates += st.fluid_candidates;
         a.boundary_candidates += st.boundary_candidates;
         a.list_waves = std::max(a.list_waves, st.list_waves);
+        a.first_density_error = std::max(a.first_density_error, st.first_density_error);
     }
     if (!any) a.converged = 0;
     return a;
@@ -1906,13 +1970,12 @@ static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
     apply_param_changes(s);
     if (out) out->assign(s->scenes.size(), pcisph_step_stats{});
     if (!s->total_fluid()) return;
-    // The warm start is left out for a scene whose last step ran to the iteration cap
-    // without converging (a start cannot shorten a loop the cap ends; the scene's step
-    // is then exactly the plain one), and out of the step when that holds for every
-    // scene (no inference, network pressure force or dual first iteration).
+    // A scene left out of the warm start (warm_start_eligible) runs exactly the plain
+    // step, and none of the inference, network pressure force and dual first iteration;
+    // the step leaves them out altogether when every scene is left out.
     WarmArgs w;
     bool any_warm = false;
-    for (const auto& sc : s->scenes) any_warm = any_warm || (sc.params.fluid_count && !sc.capped);
+    for (const auto& sc : s->scenes) any_warm = any_warm || (sc.params.fluid_count && !sc.warm_skip);
     if (s->warm_start && s->mlp && any_warm) w.params = s->mlp;
     if (s->collect_stride && s->samples_x && s->samples_count < s->samples_cap) {
         w.collect_stride = s->collect_stride;
@@ -1925,7 +1988,7 @@ static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
     std::vector<pcisph::SceneInput> in;
     bool sort_boundary = s->opt.resort_boundary != 0;
     for (auto& sc : s->scenes) {
-        in.push_back({&sc.params, sc.threshold, !sc.capped});
+        in.push_back({&sc.params, sc.threshold, w.params != nullptr && !sc.warm_skip});
         sort_boundary = sort_boundary || sc.boundary_updated;
     }
     std::vector<StepResult> res;
@@ -1934,9 +1997,13 @@ static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
     s->samples_count = std::min(s->samples_cap, s->samples_count + samples);
     for (auto& sc : s->scenes) sc.boundary_updated = false;
     for (size_t i = 0; i < res.size() && i < s->scenes.size(); ++i)
-        s->scenes[i].capped = res[i].iterations >= s->opt.max_iterations && !res[i].converged;
+        s->scenes[i].warm_skip =
+            !warm_start_eligible(res[i], s->opt, s->scenes[i].params.rest_density, s->scenes[i].threshold);
     if (out)
-        for (size_t i = 0; i < res.size(); ++i) (*out)[i] = to_stats(s, res[i], s->scenes[i].params);
+        for (size_t i = 0; i < res.size(); ++i) {
+            (*out)[i] = to_stats(s, res[i], s->scenes[i].params);
+            if (s->scenes[i].params.fluid_count && s->scenes[i].warm_skip) (*out)[i].warm_start |= 4u;
+        }
 }
 
 static void mark_nonfinite(pcisph_solver* s, std::vector<pcisph_step_stats>& v) {
@@ -2196,7 +2263,7 @@ int pcisph_set_warm_start(pcisph_solver* s, int enable) {
     return guarded(s, [&] {
         if (enable) s->ensure_mlp();
         s->warm_start = enable != 0;
-        for (auto& sc : s->scenes) sc.capped = false;
+        for (auto& sc : s->scenes) sc.warm_skip = false;
     });
 }
 
@@ -2205,7 +2272,7 @@ int pcisph_set_warm_start(pcisph_solver* s, int enable) {
 // kernels and each radix digit width, from the same starting state each time (a
 // device snapshot restored after every trial), then keeps the fastest pair in the
 // solver options.  The state after the call is the state before it: particle state,
-// pending boundary sorts, each scene's warm-start eligibility (its capped flag) and
+// pending boundary sorts, each scene's warm-start eligibility and
 // sample collection; on an error the options are the caller's again too.
 int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen) {
     return guarded(s, [&] {
@@ -2213,15 +2280,15 @@ int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_option
         if (!trial_steps) trial_steps = 3;
         apply_param_changes(s);
         // the boundary sort a pending update would do is part of every trial's first step
-        std::vector<bool> boundary_pending, capped;
+        std::vector<bool> boundary_pending, warm_skip;
         for (auto& sc : s->scenes) {
             boundary_pending.push_back(sc.boundary_updated);
-            capped.push_back(sc.capped);
+            warm_skip.push_back(sc.warm_skip);
         }
         auto reset_boundary = [&] {
             for (size_t i = 0; i < s->scenes.size(); ++i) {
                 s->scenes[i].boundary_updated = boundary_pending[i];
-                s->scenes[i].capped = capped[i];
+                s->scenes[i].warm_skip = warm_skip[i];
             }
         };
         s->eng->snapshot();

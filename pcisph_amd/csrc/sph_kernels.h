@@ -75,7 +75,8 @@ struct SphParams {
     uint32_t box_lo[3];              // cell box of the dense range grids (CellTable); box_dim[0] == 0: none
     uint32_t box_dim[3];
     uint32_t box_base;               // its first entry in the batch's grids
-    uint32_t warm;                   // 1: a warm-started step starts this scene from the network
+    uint32_t warm;                   // 1: this step starts this scene from the network (a warm-started
+                                     // step, the scene not left out of it); 0 in every other step
 };
 
 // One workgroup's share of a launch: particles [begin, end) of one scene.
@@ -132,6 +133,7 @@ struct SceneResult {
     unsigned long long max_dv;   // bits of the max density variation (precision R)
     unsigned long long max_exp;  // bits of the max expansion
     unsigned long long candidates[2];
+    unsigned long long first_dv;  // bits of the plain start's max density variation after iteration 0
     unsigned int iterations;
     unsigned int warm_rejected;
     unsigned int list_waves;
@@ -147,6 +149,7 @@ struct SceneResult {
 struct FeatureArgs {
     float* feat;          // float[n][8] or null (no features)
     const uint4* counts;  // per-particle neighbour counts of the density pass
+    uint32_t all;         // 1: every scene's (samples are collected), 0: the warm-started scenes' only
 };
 
 // The batch as every step kernel sees it.  Neighbour lists and their counts are
@@ -240,6 +243,14 @@ __device__ inline bool iteration_skipped(const LoopState* st, uint32_t k, const 
     if (k == 0) return false;
     return iteration_is_last<R>(st->max_dv, k - 1, P.min_iterations, P.max_iterations, P.rho0,
                                 P.threshold);
+}
+
+// The pressure pass of iteration k leaves a scene out when an earlier iteration ended
+// its loop, and at iteration 0 when the step starts it from the network: the dual
+// pass (k_pressure_dual) runs that iteration for it.
+template <typename R>
+__device__ inline bool pressure_pass_skipped(const LoopState* st, uint32_t k, const SphParams<R>& P) {
+    return (k == 0 && P.warm) || iteration_skipped<R>(st, k, P);
 }
 
 // ------------------------------------------------------------- arithmetic modes
@@ -902,7 +913,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, con
         bucket = P.bucket_base + hash_key((uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z, P.grid);
         R vx = 0, vy = 0, vz = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
         float a_rho = 0.f, a_p = 0.f, a_div = 0.f, a_n = 0.f;
-        const bool want = F.feat != nullptr;
+        const bool want = F.feat != nullptr && (F.all || P.warm);
         struct Cand {
             V4<R> o, ov, on;
         };
@@ -993,7 +1004,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
-    if (iteration_skipped<R>(st, iter, P)) return;
+    if (pressure_pass_skipped<R>(st, iter, P)) return;
     if (threadIdx.x == 0) s_list_waves = 0;
     const CellTable FT = cell_table(P, fcells, B.fgrid);
     const uint32_t t = rg.begin + threadIdx.x;
@@ -1188,7 +1199,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pressure_tile(
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<float> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
-    if (iteration_skipped<float>(st, iter, P)) return;
+    if (pressure_pass_skipped<float>(st, iter, P)) return;
     const uint2* fc = fcells + P.bucket_base;
     const uint2* bc = bcells + P.bucket_base;
     const uint32_t t = rg.begin + threadIdx.x;
@@ -1309,6 +1320,7 @@ __device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* 
         r.candidates[1] = st->boundary_candidates;
         r.iterations = iter + 1;
         r.warm_rejected = st->warm_rejected;
+        r.first_dv = P.warm ? st->dual_dv[1] : st->max_dv[0];
         r.list_waves = st->list_waves;
         r.running =
             iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations, P.rho0, P.threshold) ? 0u : 1u;
@@ -1350,7 +1362,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
-    if (!warm_pass && iteration_skipped<R>(st, iter, P)) return;
+    if (warm_pass ? !P.warm : iteration_skipped<R>(st, iter, P)) return;
     const bool last = !warm_pass && iteration_is_last<R>(st->max_dv, iter, P.min_iterations, P.max_iterations,
                                                          P.rho0, P.threshold);
     const uint32_t t = rg.begin + threadIdx.x;
@@ -1415,7 +1427,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
 // The network's start: p = max(0, p_prev + y p_ref), y the inference kernel's output
 // (held in p) and p_prev the particle's converged pressure of the previous step
 // (the velocity's w); then the force pass's p/rho^2.  A network that outputs 0
-// starts every particle from its previous pressure.  Fluid ranges.
+// starts every particle from its previous pressure.  Fluid ranges of the scenes
+// the step starts from the network.
 template <typename R>
 __global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, const V4<R>* __restrict__ vel,
                              R* __restrict__ p, V4<R>* __restrict__ xf) {
@@ -1423,9 +1436,8 @@ __global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, const V4<
     const uint32_t i = rg.begin + threadIdx.x;
     if (i >= rg.end) return;
     const SphParams<R>& P = B.scenes[rg.scene];
-    // a scene left out of the warm start starts from zero: its network pressure force
-    // is then exactly zero and its step exactly the plain one
-    const R pi = P.warm ? fmax(R(0), vel[i].w + (R)((float)p[i] * P.p_ref)) : R(0);
+    if (!P.warm) return;  // a scene left out of the warm start keeps the plain start
+    const R pi = fmax(R(0), vel[i].w + (R)((float)p[i] * P.p_ref));
     p[i] = pi;
     V4<R> o = xr[i];
     o.w = pi / (o.w * o.w);
@@ -1448,6 +1460,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
     if (D.disp_next) clear_table(D.disp_next, D.buckets);  // for iteration 1's prediction
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<R> P = B.scenes[rg.scene];
+    if (!P.warm) return;  // scenes left out of the warm start: the plain pass runs iteration 0
     LoopState* st = B.states + rg.scene;
     const CellTable FT = cell_table(P, fcells, B.fgrid);
     const uint32_t t = rg.begin + threadIdx.x;
@@ -1539,11 +1552,12 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
     block_max_to<4>(m, dst, lds);
 }
 
-// Per scene: picks the start with the smaller residual and records its iteration-0
-// results where the loop's termination logic reads them.
+// Per warm-started scene: picks the start with the smaller residual and records its
+// iteration-0 results where the loop's termination logic reads them.
 template <typename R>
-__global__ void k_select_start(LoopState* states, uint32_t scene_count) {
+__global__ void k_select_start(const SphParams<R>* __restrict__ scenes, LoopState* states, uint32_t scene_count) {
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < scene_count; s += gridDim.x * blockDim.x) {
+        if (!scenes[s].warm) continue;  // iteration 0 was the plain pass's
         LoopState* st = states + s;
         const R dw = Bits<R>::to(st->dual_dv[0]), ew = Bits<R>::to(st->dual_exp[0]);
         const R dz = Bits<R>::to(st->dual_dv[1]), ez = Bits<R>::to(st->dual_exp[1]);

@@ -71,7 +71,8 @@ struct SphLaunch {
     static void pressure(const Launch& L, const DeviceArrays<R>& A, uint32_t iter);
     static void pressure_force(const Launch& L, const DeviceArrays<R>& A, uint32_t iter, HostMirror* host,
                                SceneResult* res, const unsigned int* tag, int warm_pass, V4<R>* xp_save);
-    // iteration 0 of a warm-started step (k_pressure_dual, k_select_start, k_adopt_plain_start)
+    // iteration 0 of the scenes a warm-started step starts from the network (k_pressure_dual,
+    // k_select_start, k_adopt_plain_start); the pressure pass runs it for the others
     static void pressure_dual(const Launch& L, const DeviceArrays<R>& A, const DualArrays<R>& D);
     static void boundary_init(const Launch& L, const DeviceArrays<R>& A);
 };

@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
     const BlockRange rg = B.blocks[blockIdx.x];
     const SphParams<float> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
-    if (iteration_skipped<float>(st, iter, P)) return;
+    if (pressure_pass_skipped<float>(st, iter, P)) return;
     if (rg.flags & kRangeBoundary) {
         // boundary mode (pcisph.cl:231-236): binit + the fluid candidates at x^
         const uint32_t b = rg.begin + threadIdx.x;

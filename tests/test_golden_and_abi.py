@@ -45,7 +45,7 @@ def test_struct_layouts_match_header():
     assert SimulationParams.bucket_count.offset == 28
     assert SimulationParams.density_variation_scaling_factor.offset == 76
     assert ctypes.sizeof(SolverOptions) == 40
-    assert ctypes.sizeof(StepStats) == 48
+    assert ctypes.sizeof(StepStats) == 56 and StepStats.first_density_error.offset == 48
     text = open(os.path.join(ROOT, "include", "pcisph.h")).read()
     for f in Field:
         assert re.search(rf"PCISPH_{f.name}\s*=\s*{int(f)}\b", text), f.name

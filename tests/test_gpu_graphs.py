@@ -6,6 +6,8 @@ in device memory, so every replay publishes the step that runs it.  Each test ru
 the same scene both ways and compares every step's statistics and, after every step
 (downloads and synchronisations between the replays), the particle state bit for bit.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -73,12 +75,17 @@ def test_graph_matches_oracle(gpu, oracle_lib):
         np.testing.assert_array_equal(rg[name], ro[name], err_msg=name)
 
 
-def test_graph_batch_warm_start_equal_eager(gpu):
+@pytest.mark.parametrize("network", ["random", "trained"])
+def test_graph_batch_warm_start_equal_eager(gpu, network):
     a, b = small_scenes()
     batch = [a, b, scenes.scaled_sphere_drop(1500, seed=5)]
-    params = init_params(3)
-    params[-1] = 0.5
-    print("case graphs batch warm start", flush=True)
+    if network == "random":
+        params = init_params(3)
+        params[-1] = 0.5
+    else:  # the committed network, whose starts these scenes keep
+        params = np.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pcisph_amd",
+                                      "data", "warm_start.npy"))
+    print(f"case graphs batch warm start {network}", flush=True)
     runs = []
     for chunk in (0, 4):
         g = gpu.Fluid(precision=gpu.PRECISION_FAST)

@@ -174,10 +174,13 @@ def test_warm_start_fallback_is_the_plain_loop(gpu, oracle_lib, b3):
     g.set_warm_start(True)
     o = oracle_lib.OracleFluid()
     o.load_scene(scene, max_iterations=20)
-    for _ in range(3):
+    for k in range(3):
         a, b = g.step(), o.step()
-        assert a["warm_start"] == 3, a
+        # the network's start is evaluated and rejected, unless the step before left
+        # the scene out (bit 2: it converged at the minimum with the plain start)
+        assert a["warm_start"] & 3 == (3 if k == 0 or not prev & 4 else 0), a
         assert a["iterations"] == b["iterations"]
+        prev = a["warm_start"]
     for f in (Field.FLUID_POSITIONS, Field.FLUID_VELOCITIES, Field.FLUID_PRESSURES, Field.FLUID_DENSITIES):
         np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
 
