@@ -429,7 +429,12 @@ class Engine final : public EngineBase {
         dfree(stage_rows_);
         dfree(stage_src_);
         dfree(stage_info_);
-        stage_n_ = stage_tiles_ = 0;
+        for (int k = 0; k < 2; ++k) {
+            dfree(bstage_keys_[k]);
+            dfree(bstage_vals_[k]);
+        }
+        stage_n_ = stage_tiles_ = stage_nb_ = 0;
+        bwork_key_.clear();
     }
 
     // New arrays of `count` elements holding each scene's old contents (at most the
@@ -1002,6 +1007,7 @@ class Engine final : public EngineBase {
         put(L.block), put(fast), put(warm), put(features), put(all_features), put(digit), put(key_bits), put(maxit),
             put(chunk);
         put(staged), ptr(A_.stage.work), ptr(A_.stage.rows), ptr(A_.stage.src), ptr(A_.stage.info);
+        ptr(A_.stage.bwork);
         put(layout_bits), put(morton_bits);
         put(A_.tile_pressure);
         put(A_.n), put(A_.nb), put(A_.B);
@@ -1075,7 +1081,7 @@ class Engine final : public EngineBase {
         if (staged) {
             ensure_stage_buffers(fluid_blocks_);
             A_.stage = StageMaps{stage_vals_[radix_sort_result(A_.n, stage_bits, digit)], stage_rows_, stage_src_,
-                                 stage_info_};
+                                 stage_info_, bstage_vals_[radix_sort_result(A_.nb, stage_bits, digit)]};
         }
         if (features) ensure_warm_buffers(warm);
         if (warm) upload_warm_rows(scenes);
@@ -1116,6 +1122,17 @@ class Engine final : public EngineBase {
             DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.bcells, A_.bgrid, A_.bpos);
             HIP_CHECK(hipGetLastError());
             bgrid_stale_ = false;
+        }
+        if (sort_boundary) bwork_key_.clear();
+        if (staged && A_.nb) {
+            const std::vector<uint64_t> bk{digit, stage_bits, A_.nb, (uint64_t)(uintptr_t)A_.stage.bwork};
+            if (bk != bwork_key_) {
+                const int r = DataLaunch<R>::boundary_order(L, A_, obits, stage_bits, digit, sort_scratch_,
+                                                            bstage_keys_, bstage_vals_);
+                if (r < 0 || bstage_vals_[r] != A_.stage.bwork)
+                    throw Error(PCISPH_ERR_INTERNAL, "boundary work order not where expected");
+                bwork_key_ = bk;
+            }
         }
         const LaunchTable ops = launch(fast);
         // everything before the correction loop
@@ -1182,7 +1199,12 @@ class Engine final : public EngineBase {
         const DeviceArrays<R>* AI = &A_;
         DeviceArrays<R> AR = A_;
         size_t table_blocks = blocks_host_.size();
+        bool recut = false;
         auto keep_running = [&]() {
+            // the host copies of earlier tables: the stream has run past their copies
+            // once an iteration of this step has been seen
+            if (!recut) running_host_.clear();
+            recut = true;
             std::vector<BlockRange> t;
             uint32_t nf = 0;
             for (const BlockRange& b : blocks_host_)
@@ -1191,14 +1213,17 @@ class Engine final : public EngineBase {
                     nf += (b.flags & kRangeBoundary) ? 0u : 1u;
                 }
             if (t.empty() || t.size() * 2 > table_blocks) return;
-            // stream-ordered: iterations enqueued before read the previous table first
-            HIP_CHECK(hipMemcpyAsync(running_table_, t.data(), sizeof(BlockRange) * t.size(),
+            // stream-ordered: iterations enqueued before read the previous table first;
+            // the host copy stays alive until the copy has run
+            running_host_.push_back(std::move(t));
+            const std::vector<BlockRange>& h = running_host_.back();
+            HIP_CHECK(hipMemcpyAsync(running_table_, h.data(), sizeof(BlockRange) * h.size(),
                                      hipMemcpyHostToDevice, stream_));
             AR.batch.blocks = running_table_;
             AR.batch.fluid_blocks = nf;
-            AR.batch.boundary_blocks = (uint32_t)t.size() - nf;
+            AR.batch.boundary_blocks = (uint32_t)h.size() - nf;
             AI = &AR;
-            table_blocks = t.size();
+            table_blocks = h.size();
         };
         auto iteration = [&](uint32_t it) {
             const DeviceArrays<R>& A = *AI;
@@ -1384,6 +1409,14 @@ class Engine final : public EngineBase {
             dalloc(stage_info_, std::max(tiles, 1u), stream_);
             stage_tiles_ = tiles;
         }
+        if (stage_nb_ != A_.nb || !bstage_vals_[0]) {
+            for (int k = 0; k < 2; ++k) {
+                dalloc(bstage_keys_[k], std::max(A_.nb, 1u), stream_);
+                dalloc(bstage_vals_[k], std::max(A_.nb, 1u), stream_);
+            }
+            stage_nb_ = A_.nb;
+            bwork_key_.clear();
+        }
     }
 
     // Buffers of the Morton layout: its sort, the bucket codes, the reference's
@@ -1498,6 +1531,7 @@ class Engine final : public EngineBase {
     SphParams<R>* scene_table_ = nullptr;
     BlockRange* block_table_ = nullptr;
     BlockRange* running_table_ = nullptr;  // the ranges of the scenes still running (batches)
+    std::vector<std::vector<BlockRange>> running_host_;  // its uploads of the current step
     uint32_t* flags_dev_ = nullptr;  // per-scene non-finite flags, gathered
     size_t flags_cap_ = 0;
     std::vector<BlockRange> blocks_host_;
@@ -1535,7 +1569,12 @@ class Engine final : public EngineBase {
     uint32_t* stage_rows_ = nullptr;
     uint32_t* stage_src_ = nullptr;
     uint32_t* stage_info_ = nullptr;
-    uint32_t stage_n_ = 0, stage_tiles_ = 0;
+    // the boundary work order of the staged pass, sorted when the boundary is (or the
+    // sort's digit width or code bits change): keys and values of its sort
+    uint32_t* bstage_keys_[2] = {nullptr, nullptr};
+    uint32_t* bstage_vals_[2] = {nullptr, nullptr};
+    std::vector<uint64_t> bwork_key_;
+    uint32_t stage_n_ = 0, stage_tiles_ = 0, stage_nb_ = 0;
     uint32_t* okeys_[2] = {nullptr, nullptr};  // layout sort: bucket codes
     uint32_t* ovals_[2] = {nullptr, nullptr};  // and canonical indices (the result is the layout)
     uint32_t* canon_ = nullptr;                // slot -> canonical index of the public fluid arrays

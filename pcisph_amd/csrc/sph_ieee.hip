@@ -279,6 +279,16 @@ void DataLaunch<R>::stage_setup(const Launch& L, const DeviceArrays<R>& A, uint3
 }
 
 template <typename R>
+int DataLaunch<R>::boundary_order(const Launch& L, const DeviceArrays<R>& A, uint32_t bits, uint32_t key_bits,
+                                  uint32_t digit_bits, uint32_t* scratch, uint32_t* keys[2], uint32_t* vals[2]) {
+    const Batch<R>& B = A.batch;
+    if (!B.boundary_blocks) return 0;
+    hipLaunchKernelGGL((k_stage_codes<R>), dim3(B.boundary_blocks), dim3(L.block), 0, L.stream, B.scenes,
+                       B.blocks + B.fluid_blocks, A.bpos, bits, keys[0], vals[0]);
+    return radix_sort(L, keys, vals, B.nb_total, key_bits, digit_bits, scratch);
+}
+
+template <typename R>
 void DataLaunch<R>::warm_apply(const Launch& L, const DeviceArrays<R>& A) {
     if (!A.batch.fluid_blocks) return;
     hipLaunchKernelGGL((k_warm_apply<R>), dim3(A.batch.fluid_blocks), dim3(L.block), 0, L.stream, A.batch, A.xr,

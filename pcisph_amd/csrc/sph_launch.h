@@ -146,6 +146,11 @@ struct DataLaunch {
     // row descriptors into A.stage (whose `work` must be vals[radix_sort_result(...)])
     static void stage_setup(const Launch& L, const DeviceArrays<R>& A, uint32_t bits, uint32_t key_bits,
                             uint32_t digit_bits, uint32_t* scratch, uint32_t* keys[2], uint32_t* vals[2]);
+    // the boundary work order of the staged pressure pass: the boundary slots sorted
+    // stably by the Morton code of their own cell (as stage_setup's work order) through
+    // keys/vals; returns the index of the buffer pair holding it (radix_sort)
+    static int boundary_order(const Launch& L, const DeviceArrays<R>& A, uint32_t bits, uint32_t key_bits,
+                              uint32_t digit_bits, uint32_t* scratch, uint32_t* keys[2], uint32_t* vals[2]);
 };
 
 void fill_u32(const Launch& L, uint32_t* p, size_t n, uint32_t value);

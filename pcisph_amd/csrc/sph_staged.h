@@ -21,7 +21,9 @@
 // within each bucket) and every lane walks its nine rows there: the reference's
 // candidates in the reference's order (rows dz-major then dy, x fastest within a row,
 // ascending index within a bucket), so sums are unchanged bit for bit.  Boundary
-// candidates are walked from global memory as in k_pressure.  A workgroup whose
+// candidates are walked from global memory as in k_pressure.  The boundary ranges of
+// the same launch (boundary mode) take their particles in the Morton order of their
+// own cells (the boundary work order, sorted when the boundary is).  A workgroup whose
 // staging does not fit (kStageCap candidates, kStageMaxCells cells, kStageLines lines,
 // cell coordinates of magnitude 2^29 or more) walks global memory instead.
 #pragma once
@@ -44,6 +46,7 @@ struct StageMaps {
     uint32_t* rows;        // [9][n_total] per work index: row r's LDS start | length << 16
     uint32_t* src;         // [workgroups][kStageCap] slot of each staged candidate
     uint32_t* info;        // [workgroups] staged candidate count, or kStageFallback
+    const uint32_t* bwork; // [nb_total] boundary work index -> boundary slot (Morton order of own cells)
 };
 
 // The bucket range of cell (x, y, z) as for_each_range_row reads it (grid flags removed).
@@ -80,7 +83,7 @@ __device__ __forceinline__ uint2 block_scan2(uint32_t a, uint32_t b, uint2& tota
     return make_uint2(ba + xa - a, bb + xb - b);
 }
 
-// Code of a fluid particle's own cell: scene index above the Morton code of the cell
+// Code of a particle's own cell: scene index above the Morton code of the cell
 // coordinates modulo 2^bits (the Morton layout's code of buckets, taken per particle).
 __device__ __forceinline__ uint32_t stage_spread3(uint32_t v, uint32_t bits) {
     uint32_t out = 0;
@@ -263,9 +266,11 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
 
 // update_pressure (pcisph.cl:212-263), the fluid ranges with their candidates from the
 // workgroup's LDS staging (see above) -- the same results as k_pressure's bucket walk --
-// and the boundary ranges (boundary mode) walking their fluid buckets in global memory,
-// in one launch so both kinds of workgroups share the device.  Fluid `blocks` entries
-// carry their staging index (kRangeTileShift).
+// and the boundary ranges (boundary mode) walking their fluid buckets in global
+// memory, in one launch so both kinds of workgroups share the device.  Fluid `blocks`
+// entries carry their staging index (kRangeTileShift); boundary ranges are taken in
+// the boundary work order (S.bwork), so a wavefront's 27-cell neighbourhoods overlap
+// instead of lying wherever the hash put their buckets.
 template <bool FAST>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
     Batch<float> B, uint32_t iter, const uint2* __restrict__ bcells, const float4* __restrict__ bpos,
@@ -282,8 +287,9 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
     if (pressure_pass_skipped<float>(st, iter, P)) return;
     if (rg.flags & kRangeBoundary) {
         // boundary mode (pcisph.cl:231-236): binit + the fluid candidates at x^
-        const uint32_t b = rg.begin + threadIdx.x;
-        if (b < rg.end) {
+        const uint32_t t = rg.begin + threadIdx.x;
+        if (t < rg.end) {
+            const uint32_t b = S.bwork[t];
             const float4 self = bpos[b];
             const int3 c = cell_of4<float>(self, P.cell);
             float pd = 0.f;
