@@ -108,20 +108,24 @@ __device__ __forceinline__ void for_each_bucket_row(int3 c, const HashGrid& g, F
 //
 // A grid entry may carry kFarBucket in its start word: every particle of that bucket
 // lies in a cell at least kFarCells cells from the entry's cell along some axis (hash
-// collisions from elsewhere in the scene), hence at least 1.99 cell sizes from every
-// position whose cell is the entry's cell or one of its 26 neighbours -- the positions
-// that look the entry up (the 0.01 covers the rounding of cell coordinates of
-// magnitude below 2^15, the only ones flagged).  A walk whose evaluation points stay within d of the
-// walking particle's position, and whose terms vanish beyond radius rho, adds exactly
-// zero for such a bucket when d + rho <= kFarReach cell sizes: it sets `prune` and
-// reads those entries as empty.  Candidate counts still include them.
+// collisions from elsewhere in the scene), hence at least kFarCells - 2 - 0.01 cell
+// sizes from every position whose cell is the entry's cell or one of its 26
+// neighbours -- the positions that look the entry up (the 0.01 covers the rounding of
+// cell coordinates of magnitude below 2^15, the only ones flagged).  A walk whose
+// evaluation points stay within d of the walking particle's position, and whose terms
+// vanish beyond radius rho, adds exactly zero for such a bucket when d + rho <=
+// kFarReach cell sizes: it sets `prune` and reads those entries as empty.  Candidate
+// counts still include them.  Eight cells keep a walk prunable while its particle's
+// predicted position is up to ~5 cells from where the step began (the start-up
+// transient of the dam breaks moves 6-20% of the particles by more than one cell
+// within a step), and hash collisions from closer than that are rare.
 //
 // kQuietCell in an entry's end word marks a cell all 27 of whose neighbourhood
 // entries are empty or far: a pruning walk from it has nothing to visit.
 constexpr uint32_t kFarBucket = 0x80000000u;
 constexpr uint32_t kQuietCell = 0x80000000u;
-constexpr int kFarCells = 4;
-constexpr float kFarReach = 1.98f;
+constexpr int kFarCells = 8;
+constexpr float kFarReach = (float)kFarCells - 2.02f;
 
 struct CellTable {
     const uint2* cells;

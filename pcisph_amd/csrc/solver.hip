@@ -429,6 +429,8 @@ class Engine final : public EngineBase {
         dfree(stage_rows_);
         dfree(stage_src_);
         dfree(stage_info_);
+        dfree(stage_brows_);
+        dfree(stage_binfo_);
         for (int k = 0; k < 2; ++k) {
             dfree(bstage_keys_[k]);
             dfree(bstage_vals_[k]);
@@ -1007,7 +1009,7 @@ class Engine final : public EngineBase {
         put(L.block), put(fast), put(warm), put(features), put(all_features), put(digit), put(key_bits), put(maxit),
             put(chunk);
         put(staged), ptr(A_.stage.work), ptr(A_.stage.rows), ptr(A_.stage.src), ptr(A_.stage.info);
-        ptr(A_.stage.bwork);
+        ptr(A_.stage.bwork), ptr(A_.stage.brows), ptr(A_.stage.binfo);
         put(layout_bits), put(morton_bits);
         put(A_.tile_pressure);
         put(A_.n), put(A_.nb), put(A_.B);
@@ -1080,8 +1082,13 @@ class Engine final : public EngineBase {
         A_.staged_pressure = staged;
         if (staged) {
             ensure_stage_buffers(fluid_blocks_);
-            A_.stage = StageMaps{stage_vals_[radix_sort_result(A_.n, stage_bits, digit)], stage_rows_, stage_src_,
-                                 stage_info_, bstage_vals_[radix_sort_result(A_.nb, stage_bits, digit)]};
+            A_.stage = StageMaps{stage_vals_[radix_sort_result(A_.n, stage_bits, digit)],
+                                 stage_rows_,
+                                 stage_src_,
+                                 stage_info_,
+                                 stage_brows_,
+                                 stage_binfo_,
+                                 bstage_vals_[radix_sort_result(A_.nb, stage_bits, digit)]};
         }
         if (features) ensure_warm_buffers(warm);
         if (warm) upload_warm_rows(scenes);
@@ -1402,11 +1409,13 @@ class Engine final : public EngineBase {
                 dalloc(stage_vals_[k], std::max(A_.n, 1u), stream_);
             }
             dalloc(stage_rows_, 9 * (size_t)std::max(A_.n, 1u), stream_);
+            dalloc(stage_brows_, 9 * (size_t)std::max(A_.n, 1u), stream_);
             stage_n_ = A_.n;
         }
         if (stage_tiles_ < tiles || !stage_src_) {
             dalloc(stage_src_, (size_t)std::max(tiles, 1u) * kStageCap, stream_);
             dalloc(stage_info_, std::max(tiles, 1u), stream_);
+            dalloc(stage_binfo_, std::max(tiles, 1u), stream_);
             stage_tiles_ = tiles;
         }
         if (stage_nb_ != A_.nb || !bstage_vals_[0]) {
@@ -1563,12 +1572,14 @@ class Engine final : public EngineBase {
     std::vector<uint8_t> near_;          // per scene: lists of radius (1 + kNearSkin) h this step
     std::vector<uint32_t> near_steps_;   // steps since the scene went to near lists
     // staging of the FP32 pressure pass (sph_staged.h): work-order sort, row
-    // descriptors [9][n], staging maps [fluid ranges][kStageCap], staged counts
+    // descriptors [9][n] (fluid and boundary rows), staging maps [fluid ranges][kStageCap], staged counts
     uint32_t* stage_keys_[2] = {nullptr, nullptr};
     uint32_t* stage_vals_[2] = {nullptr, nullptr};
     uint32_t* stage_rows_ = nullptr;
     uint32_t* stage_src_ = nullptr;
     uint32_t* stage_info_ = nullptr;
+    uint32_t* stage_brows_ = nullptr;
+    uint32_t* stage_binfo_ = nullptr;
     // the boundary work order of the staged pass, sorted when the boundary is (or the
     // sort's digit width or code bits change): keys and values of its sort
     uint32_t* bstage_keys_[2] = {nullptr, nullptr};

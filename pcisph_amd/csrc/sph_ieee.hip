@@ -274,8 +274,8 @@ void DataLaunch<R>::stage_setup(const Launch& L, const DeviceArrays<R>& A, uint3
     hipLaunchKernelGGL((k_stage_codes<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B.scenes, B.blocks,
                        A.pos, bits, keys[0], vals[0]);
     radix_sort(L, keys, vals, B.n_total, key_bits, digit_bits, scratch);
-    hipLaunchKernelGGL((k_stage_setup<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B, A.cells, A.pos,
-                       A.stage);
+    hipLaunchKernelGGL((k_stage_setup<R>), dim3(B.fluid_blocks), dim3(L.block), 0, L.stream, B, A.cells, A.bcells,
+                       A.pos, A.stage);
 }
 
 template <typename R>

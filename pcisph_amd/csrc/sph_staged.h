@@ -20,8 +20,12 @@
 // Each iteration a workgroup copies its staged predicted positions into LDS (coalesced
 // within each bucket) and every lane walks its nine rows there: the reference's
 // candidates in the reference's order (rows dz-major then dy, x fastest within a row,
-// ascending index within a bucket), so sums are unchanged bit for bit.  Boundary
-// candidates are walked from global memory as in k_pressure.  The boundary ranges of
+// ascending index within a bucket), so sums are unchanged bit for bit.  The boundary
+// candidates of the same cells are staged after the fluid ones, as a pruning walk
+// reads them (buckets flagged far left out): a particle whose walk may prune
+// (CellTable) takes its nine boundary rows from LDS, the others walk global memory
+// as in k_pressure; far buckets add exactly zero for the former, so both give the
+// walk's sum.  The boundary ranges of
 // the same launch (boundary mode) take their particles in the Morton order of their
 // own cells (the boundary work order, sorted when the boundary is).  A workgroup whose
 // staging does not fit (kStageCap candidates, kStageMaxCells cells, kStageLines lines,
@@ -44,18 +48,25 @@ constexpr int kRangeBatch = 4;  // cell-range lookups in flight per thread (setu
 struct StageMaps {
     const uint32_t* work;  // [n_total] work index -> fluid slot (per-step Morton order of own cells)
     uint32_t* rows;        // [9][n_total] per work index: row r's LDS start | length << 16
-    uint32_t* src;         // [workgroups][kStageCap] slot of each staged candidate
-    uint32_t* info;        // [workgroups] staged candidate count, or kStageFallback
+    uint32_t* src;         // [workgroups][kStageCap] slot of each staged candidate (fluid, then boundary)
+    uint32_t* info;        // [workgroups] staged fluid candidate count, or kStageFallback
+    uint32_t* brows;       // [9][n_total] per work index: boundary row r's LDS start | length << 16
+    uint32_t* binfo;       // [workgroups] staged boundary candidate count, or kStageFallback
     const uint32_t* bwork; // [nb_total] boundary work index -> boundary slot (Morton order of own cells)
 };
 
-// The bucket range of cell (x, y, z) as for_each_range_row reads it (grid flags removed).
-__device__ __forceinline__ uint2 cell_range(const CellTable& T, uint32_t x, uint32_t y, uint32_t z) {
+// The bucket range of cell (x, y, z) as for_each_range_row reads it (grid flags
+// removed); with `prune`, a grid entry flagged kFarBucket reads as empty, as a pruning
+// walk reads it.
+__device__ __forceinline__ uint2 cell_range(const CellTable& T, uint32_t x, uint32_t y, uint32_t z,
+                                            bool prune = false) {
     const uint32_t rx = x - T.lo[0], ry = y - T.lo[1], rz = z - T.lo[2];
     if (rx < T.dim[0] && ry < T.dim[1] && rz < T.dim[2]) {
         uint2 r = T.grid[rx + T.dim[0] * (ry + T.dim[1] * rz)];
-        r.x &= ~kFarBucket;
+        const uint32_t far = r.x & kFarBucket;
+        r.x ^= far;
         r.y &= ~kQuietCell;
+        if (far && prune) r.y = r.x;
         return r;
     }
     return T.cells[hash_key(x, y, z, T.g)];
@@ -113,17 +124,21 @@ __global__ void k_stage_codes(const SphParams<R>* __restrict__ scenes, const Blo
 // of LDS.
 template <typename R>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, const uint2* __restrict__ fcells,
+                                                                  const uint2* __restrict__ bcells,
                                                                   const V4<R>* __restrict__ pos, StageMaps S) {
     __shared__ int s_lx[kStageLines], s_hx[kStageLines];  // x extent of each line
     __shared__ uint32_t s_cell0[kStageLines + 1];         // first staged cell of each line
     __shared__ uint32_t s_start[kStageMaxCells];          // bucket start of each staged cell
     __shared__ uint32_t s_off[kStageMaxCells + 1];        // staged offset of each cell (count before the scan)
+    __shared__ uint32_t s_bstart[kStageMaxCells];         // the same for the boundary buckets
+    __shared__ uint32_t s_boff[kStageMaxCells + 1];
     __shared__ uint32_t s_part[2][kMaxBlockThreads / 64];
     __shared__ int s_box[6][kMaxBlockThreads / 64];
     const BlockRange rg = B.blocks[blockIdx.x];
     const uint32_t tile = rg.flags >> kRangeTileShift;
     const SphParams<R>& P = B.scenes[rg.scene];
     const CellTable FT = cell_table(P, fcells, B.fgrid);
+    const CellTable BT = cell_table(P, bcells, B.bgrid);
     const uint32_t k = rg.begin + threadIdx.x;
     const bool act = k < rg.end;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = (int)(blockDim.x >> 6);
@@ -157,7 +172,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
     const uint32_t by = fits ? (uint32_t)(hi[1] - lo[1] + 3) : 0u, bz = fits ? (uint32_t)(hi[2] - lo[2] + 3) : 0u;
     const uint64_t lines = (uint64_t)by * bz;
     if (!fits || lines > kStageLines) {
-        if (threadIdx.x == 0) S.info[tile] = kStageFallback;
+        if (threadIdx.x == 0) S.info[tile] = S.binfo[tile] = kStageFallback;
         return;
     }
     const uint32_t nl = (uint32_t)lines;
@@ -193,19 +208,21 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
         }
         if (threadIdx.x == 0) s_cell0[nl] = tot.x;
         if (tot.x > kStageMaxCells) {
-            if (threadIdx.x == 0) S.info[tile] = kStageFallback;
+            if (threadIdx.x == 0) S.info[tile] = S.binfo[tile] = kStageFallback;
             return;
         }
     }
     __syncthreads();
     const uint32_t ncells = s_cell0[nl];
-    // 4. each staged cell's bucket range (kRangeBatch lookups in flight per thread)
+    // 4. each staged cell's fluid bucket range and its boundary bucket range as a
+    // pruning walk reads it (kRangeBatch lookups of each in flight per thread)
+    const bool walls = P.nb != 0;
     for (uint32_t q0 = threadIdx.x; q0 < ncells; q0 += kRangeBatch * blockDim.x) {
-        uint2 r[kRangeBatch];
+        uint2 r[kRangeBatch], rb[kRangeBatch];
 #pragma unroll
         for (int u = 0; u < kRangeBatch; ++u) {
             const uint32_t q = q0 + u * blockDim.x;
-            r[u] = make_uint2(0u, 0u);
+            r[u] = rb[u] = make_uint2(0u, 0u);
             if (q >= ncells) continue;
             uint32_t a = 0, b = nl;  // the last line whose first cell is <= q (a non-empty one)
             while (b - a > 1) {
@@ -213,8 +230,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
                 if (s_cell0[mid] <= q) a = mid;
                 else b = mid;
             }
-            const int x = s_lx[a] + (int)(q - s_cell0[a]);
-            r[u] = cell_range(FT, (uint32_t)x, (uint32_t)(y0 + (int)(a % by)), (uint32_t)(z0 + (int)(a / by)));
+            const uint32_t x = (uint32_t)(s_lx[a] + (int)(q - s_cell0[a]));
+            const uint32_t y = (uint32_t)(y0 + (int)(a % by)), z = (uint32_t)(z0 + (int)(a / by));
+            r[u] = cell_range(FT, x, y, z);
+            if (walls) rb[u] = cell_range(BT, x, y, z, true);
         }
 #pragma unroll
         for (int u = 0; u < kRangeBatch; ++u) {
@@ -222,28 +241,37 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
             if (q >= ncells) continue;
             s_start[q] = r[u].x;
             s_off[q] = r[u].y - r[u].x;
+            s_bstart[q] = rb[u].x;
+            s_boff[q] = rb[u].y - rb[u].x;
         }
     }
     __syncthreads();
-    // 5. staged offsets: a contiguous chunk of cells per thread, scanned
+    // 5. staged offsets of both: a contiguous chunk of cells per thread, scanned
     const uint32_t chunk = (ncells + blockDim.x - 1) / blockDim.x;
     const uint32_t q0 = min(threadIdx.x * chunk, ncells), q1 = min(q0 + chunk, ncells);
-    uint32_t sum = 0;
-    for (uint32_t q = q0; q < q1; ++q) sum += s_off[q];
+    uint32_t sum = 0, bsum = 0;
+    for (uint32_t q = q0; q < q1; ++q) sum += s_off[q], bsum += s_boff[q];
     uint2 tot;
-    const uint2 base = block_scan2(sum, 0u, tot, s_part);
+    const uint2 base = block_scan2(sum, bsum, tot, s_part);
     {
-        uint32_t run = base.x;
+        uint32_t run = base.x, brun = base.y;
         for (uint32_t q = q0; q < q1; ++q) {
-            const uint32_t cnt = s_off[q];
+            const uint32_t cnt = s_off[q], bcnt = s_boff[q];
             s_off[q] = run;
+            s_boff[q] = brun;
             run += cnt;
+            brun += bcnt;
         }
     }
     const uint32_t total = tot.x;
-    if (threadIdx.x == 0) s_off[ncells] = total;
+    // the boundary candidates follow the fluid ones in the staging, if both fit
+    const uint32_t btotal = total + tot.y <= kStageCap ? tot.y : kStageFallback;
+    if (threadIdx.x == 0) {
+        s_off[ncells] = total;
+        s_boff[ncells] = tot.y;
+    }
     if (total > kStageCap) {
-        if (threadIdx.x == 0) S.info[tile] = kStageFallback;
+        if (threadIdx.x == 0) S.info[tile] = S.binfo[tile] = kStageFallback;
         return;
     }
     __syncthreads();
@@ -252,6 +280,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
     for (uint32_t q = threadIdx.x; q < ncells; q += blockDim.x) {
         const uint32_t o = s_off[q], n = s_off[q + 1] - o, b = s_start[q];
         for (uint32_t e = 0; e < n; ++e) src[o + e] = b + e;
+        if (btotal != kStageFallback) {
+            const uint32_t bo = total + s_boff[q], bn = s_boff[q + 1] - s_boff[q], bb = s_bstart[q];
+            for (uint32_t e = 0; e < bn; ++e) src[bo + e] = bb + e;
+        }
     }
     if (act) {
         for (int row = 0; row < 9; ++row) {
@@ -259,9 +291,16 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
             const uint32_t q = s_cell0[l] + (uint32_t)(c.x - 1 - s_lx[l]);
             const uint32_t a = s_off[q], e = s_off[q + 3];
             S.rows[(size_t)row * B.n_total + k] = a | ((e - a) << 16);
+            if (btotal != kStageFallback && btotal) {
+                const uint32_t ba = total + s_boff[q], be = total + s_boff[q + 3];
+                S.brows[(size_t)row * B.n_total + k] = ba | ((be - ba) << 16);
+            }
         }
     }
-    if (threadIdx.x == 0) S.info[tile] = total;
+    if (threadIdx.x == 0) {
+        S.info[tile] = total;
+        S.binfo[tile] = btotal;
+    }
 }
 
 // update_pressure (pcisph.cl:212-263), the fluid ranges with their candidates from the
@@ -315,23 +354,29 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
     const uint32_t tile = rg.flags >> kRangeTileShift;
     const uint32_t total = S.info[tile];
     const bool staged = total != kStageFallback;
+    const uint32_t btotal = staged ? S.binfo[tile] : kStageFallback;
+    const bool bstaged = btotal != kStageFallback;
     if (staged) {
-        // kCopyBatch independent loads in flight per thread (indices, then positions)
+        // kCopyBatch independent loads in flight per thread (indices, then positions):
+        // the fluid candidates' predicted positions, then the boundary candidates'
+        const uint32_t all = total + (bstaged ? btotal : 0u);
         const uint32_t* src = S.src + (size_t)tile * kStageCap;
-        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kCopyBatch * blockDim.x) {
+        for (uint32_t q0 = threadIdx.x; q0 < all; q0 += kCopyBatch * blockDim.x) {
             uint32_t j[kCopyBatch];
             float4 v[kCopyBatch];
+            const float4* from[kCopyBatch];
 #pragma unroll
             for (int u = 0; u < kCopyBatch; ++u) {
-                const uint32_t q = q0 + u * blockDim.x;
-                j[u] = src[q < total ? q : q0];
+                const uint32_t q = q0 + u * blockDim.x, qq = q < all ? q : q0;
+                j[u] = src[qq];
+                from[u] = qq < total ? xp : bpos;
             }
 #pragma unroll
-            for (int u = 0; u < kCopyBatch; ++u) v[u] = xp[j[u]];
+            for (int u = 0; u < kCopyBatch; ++u) v[u] = from[u][j[u]];
 #pragma unroll
             for (int u = 0; u < kCopyBatch; ++u) {
                 const uint32_t q = q0 + u * blockDim.x;
-                if (q < total) {
+                if (q < all) {
                     s_x[q] = v[u].x;
                     s_y[q] = v[u].y;
                     s_z[q] = v[u].z;
@@ -354,8 +399,24 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
         if (P.nb) {
             CellTable BT = cell_table(P, bcells, B.bgrid);
             BT.prune = far_prunable<float>(displacement<float>(pr, self), (float)P.h, P.cell);
-            if (!(BT.prune && quiet_cell(BT, c)))
+            if (BT.prune && quiet_cell(BT, c)) {
+                // nothing within reach
+            } else if (BT.prune && bstaged) {
+                // the pruning walk's candidates, staged after the fluid ones (none: nothing to add)
+                if (btotal) {
+                    uint32_t d[9];
+#pragma unroll
+                    for (int row = 0; row < 9; ++row) d[row] = S.brows[(size_t)row * B.n_total + k];
+#pragma unroll 1
+                    for (int row = 0; row < 9; ++row) {
+                        const uint32_t a = d[row] & 0xFFFFu;
+                        gather_range(a, a + (d[row] >> 16),
+                                     [&](uint32_t j) { return make_float4(s_x[j], s_y[j], s_z[j], 0.f); }, add);
+                    }
+                }
+            } else {
                 walk_buckets<float>(BT, c, [&](uint32_t j) { return bpos[j]; }, add);
+            }
         }
         if (staged) {
             uint32_t d[9];

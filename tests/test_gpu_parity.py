@@ -11,6 +11,7 @@ import dataclasses
 import numpy as np
 import pytest
 
+from pcisph_amd import scenes
 from pcisph_amd.abi import Field
 from tests.helpers import (FLOAT_FIELDS, INT_FIELDS, max_rel_err, run, small_scenes, snapshot)
 
@@ -240,22 +241,29 @@ def test_list_overflow_falls_back_to_buckets(gpu, oracle_lib):
         np.testing.assert_array_equal(g.download(f), o.download(f), err_msg=Field(f).name)
 
 
+@pytest.mark.parametrize("tank", ["small", "wide"])
 @pytest.mark.parametrize("fp64", [False, True])
-def test_far_collision_pruning_bit_exact(gpu, oracle_lib, fp64):
-    """Boundary buckets that hold only far hash collisions read as empty for walks
-    that cannot reach them (512 fluid particles, 256 buckets, 6558 wall particles:
-    most boundary lookups hit such buckets).  Velocities up to 120 m/s put the
-    predicted positions anywhere from 0 to 2.5 h from x, on both sides of the reach
-    limit, so both the pruned and the full walks run in the same pressure passes;
-    every field stays the oracle's bit for bit."""
-    scene = small_scenes()[0]
-    print(f"case far collision pruning fp64={fp64}", flush=True)
+def test_far_collision_pruning_bit_exact(gpu, oracle_lib, fp64, tank):
+    """Boundary buckets that hold only hash collisions from at least eight cells away
+    read as empty for walks that cannot reach them (512 fluid particles, 256 buckets).
+    In the small tank (12.5 x 11 x 10 cells, 6558 wall particles) few buckets qualify;
+    in the wide one (30 x 15 x 30 cells) most boundary lookups hit such buckets.
+    Velocities up to 120 m/s (small) or 500 m/s (wide) put the predicted positions
+    anywhere from 0 to 1.5 h or 6 h from x -- in the wide tank on both sides of the
+    reach limit (~6 cells), so both the pruned and the full walks run in the same
+    pressure passes; every field stays the oracle's bit for bit."""
+    if tank == "small":
+        scene, vmax = small_scenes()[0], 120.0
+    else:
+        scene = scenes.dam_break((8, 8, 8), 0.02, 0.001, (1.2, 0.6, 1.2), seed=1234, name="wide_tank", steps=2)
+        vmax = 500.0
+    print(f"case far collision pruning {tank} fp64={fp64}", flush=True)
     o = oracle_lib.OracleFluid(fp64=fp64)
     g = gpu.Fluid(precision=gpu.PRECISION_FP64 if fp64 else gpu.PRECISION_IEEE)
     o.load_scene(scene, max_iterations=20)
     g.load_scene(scene, max_iterations=20)
     rng = np.random.default_rng(7)
-    v = rng.uniform(-120.0, 120.0, 3 * scene.fluid_count).astype(np.float32)
+    v = rng.uniform(-vmax, vmax, 3 * scene.fluid_count).astype(np.float32)
     o.upload(Field.FLUID_VELOCITIES, v)
     g.upload(Field.FLUID_VELOCITIES, v)
     for _ in range(2):
