@@ -135,7 +135,11 @@ typedef struct pcisph_solver_options {
                                        their rows copied once per pass and walked in the
                                        reference's order: identical results) -- automatically
                                        when no scene's pressure passes may walk neighbour
-                                       lists in the step, always with bit 4, never with bit 5.
+                                       lists in the step, always with bit 4, never with bit 5;
+                                       bit 6: the correction loop runs as one cooperative
+                                       launch whose workgroups meet at a grid barrier between
+                                       the passes (identical results; not in steps the stage
+                                       profiler times, warm-started steps or with bit 2).
                                        Default 3 (bits 0 and 1). */
     uint32_t reserved;
 } pcisph_solver_options;

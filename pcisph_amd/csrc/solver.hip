@@ -354,6 +354,10 @@ class Engine final : public EngineBase {
         HIP_CHECK(hipHostMalloc((void**)&host_, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
         std::memset((void*)host_, 0, sizeof(HostMirror));
         HIP_CHECK(hipHostGetDevicePointer((void**)&dev_host_, (void*)host_, 0));
+        // the correction loop kernel's barrier gives up after 5 s of waiting
+        int khz = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
+        sync_timeout_ = 5000ull * (unsigned long long)(khz > 0 ? khz : 100000);
     }
 
     ~Engine() override {
@@ -387,6 +391,7 @@ class Engine final : public EngineBase {
         dfree(scene_table_);
         dfree(block_table_);
         dfree(running_table_);
+        dfree(sync_);
         dfree(flags_dev_);
         dfree(states_);
         dfree(shift_dev_);
@@ -1097,7 +1102,12 @@ class Engine final : public EngineBase {
         if (!ctl_) dalloc(ctl_, 64, stream_);
         // graphs record launches; a timed step launches them one by one (and keeps the
         // graphs for the untimed ones)
-        const uint32_t chunk = prof_.begin_step() ? 0u : o.graph_iterations;
+        const bool profiled = prof_.begin_step();
+        const uint32_t chunk = profiled ? 0u : o.graph_iterations;
+        // the correction loop as one cooperative launch (work_order bit 6): not in a
+        // step the stage profiler times pass by pass, nor with the network's dual first
+        // iteration or the tiled pressure pass
+        const bool persistent = (o.work_order & 64) && !profiled && !warm && !A_.tile_pressure;
         if (chunk) {
             std::vector<uint64_t> key =
                 launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params, staged,
@@ -1279,6 +1289,15 @@ class Engine final : public EngineBase {
                 ++enq;
             }
         };
+        if (persistent) {
+            if (!sync_) dalloc(sync_, 4, stream_);
+            fill_u32(L, sync_, 4, 0u);
+            const hipError_t e =
+                ops.correction_loop(L, A_, maxit, dev_host_, dev_results_, ctl_, GridSync{sync_, sync_timeout_});
+            if (e != hipSuccess)
+                throw Error(PCISPH_ERR_DEVICE, std::string("correction loop launch: ") + hipGetErrorString(e));
+            enq = maxit;  // every iteration is in that launch
+        }
         while (enq < maxit && enq <= first + lookahead) enqueue();
         HIP_CHECK(hipGetLastError());
         uint32_t k = first;
@@ -1286,7 +1305,7 @@ class Engine final : public EngineBase {
             wait_for_iteration(k, tag);
             if (host_->running[k] == 0) break;
             if (k + 1 >= maxit) throw Error(PCISPH_ERR_INTERNAL, "scenes still running after the last iteration");
-            if (S > 1) keep_running();
+            if (S > 1 && !persistent) keep_running();
             ++k;
             if (enq < maxit && enq <= k + lookahead) enqueue();
         }
@@ -1527,6 +1546,8 @@ class Engine final : public EngineBase {
                                const unsigned int*, int, V4<R>*);
         void (*boundary_init)(const Launch&, const DeviceArrays<R>&);
         void (*pressure_dual)(const Launch&, const DeviceArrays<R>&, const DualArrays<R>&);
+        hipError_t (*correction_loop)(const Launch&, const DeviceArrays<R>&, uint32_t, HostMirror*, SceneResult*,
+                                      const unsigned int*, const GridSync&);
     };
     static LaunchTable launch(bool fast);
 
@@ -1540,6 +1561,8 @@ class Engine final : public EngineBase {
     SphParams<R>* scene_table_ = nullptr;
     BlockRange* block_table_ = nullptr;
     BlockRange* running_table_ = nullptr;  // the ranges of the scenes still running (batches)
+    unsigned int* sync_ = nullptr;          // the correction loop kernel's barrier words
+    unsigned long long sync_timeout_ = 0;   // its longest wait, wall-clock ticks
     std::vector<std::vector<BlockRange>> running_host_;  // its uploads of the current step
     uint32_t* flags_dev_ = nullptr;  // per-scene non-finite flags, gathered
     size_t flags_cap_ = 0;
@@ -1623,10 +1646,12 @@ Engine<float>::LaunchTable Engine<float>::launch(bool fast) {
     if (fast)
         return {SphLaunch<float, true>::density, SphLaunch<float, true>::normal, SphLaunch<float, true>::force_init,
                 SphLaunch<float, true>::pressure, SphLaunch<float, true>::pressure_force,
-                SphLaunch<float, true>::boundary_init, SphLaunch<float, true>::pressure_dual};
+                SphLaunch<float, true>::boundary_init, SphLaunch<float, true>::pressure_dual,
+                SphLaunch<float, true>::correction_loop};
     return {SphLaunch<float, false>::density, SphLaunch<float, false>::normal, SphLaunch<float, false>::force_init,
             SphLaunch<float, false>::pressure, SphLaunch<float, false>::pressure_force,
-            SphLaunch<float, false>::boundary_init, SphLaunch<float, false>::pressure_dual};
+            SphLaunch<float, false>::boundary_init, SphLaunch<float, false>::pressure_dual,
+            SphLaunch<float, false>::correction_loop};
 }
 
 template <>
@@ -1634,7 +1659,7 @@ Engine<double>::LaunchTable Engine<double>::launch(bool) {
     return {SphLaunch<double, false>::density, SphLaunch<double, false>::normal,
             SphLaunch<double, false>::force_init, SphLaunch<double, false>::pressure,
             SphLaunch<double, false>::pressure_force, SphLaunch<double, false>::boundary_init,
-            SphLaunch<double, false>::pressure_dual};
+            SphLaunch<double, false>::pressure_dual, SphLaunch<double, false>::correction_loop};
 }
 
 }  // namespace pcisph
@@ -1855,7 +1880,7 @@ int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "sort_digit_bits must be in 4..8");
         if (!(opt->neighbor_skin >= 0.f && opt->neighbor_skin <= 2.f))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "neighbor_skin must be in [0, 2]");
-        if (opt->work_order > 63) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..63");
+        if (opt->work_order > 127) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..127");
         s->opt = *opt;
     });
 }

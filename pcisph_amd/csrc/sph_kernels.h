@@ -991,17 +991,18 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_force_init(Batch<R> B, con
 // Distances are taken at the predicted positions, so a wavefront walks the step's
 // lists (built at x with radius (1+skin)h) only where every lane's displacement test
 // passes (NeighborLists), else its 27 buckets.
+// One workgroup's range `range` of the block table (k_pressure, k_correction_loop).
 template <typename R, bool FAST>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
-                           const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
-                           const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
-                           const V4<R>* __restrict__ xp, R* __restrict__ dv, R* __restrict__ p,
-                           V4<R>* __restrict__ xf, R* __restrict__ bp, V4<R>* __restrict__ bxf, NeighborLists NL) {
+__device__ __forceinline__ void pressure_range(const Batch<R>& B, uint32_t range, uint32_t iter,
+                                               const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
+                                               const R* __restrict__ binit, const uint2* __restrict__ fcells,
+                                               const V4<R>* __restrict__ xr, const V4<R>* __restrict__ xp,
+                                               R* __restrict__ dv, R* __restrict__ p, V4<R>* __restrict__ xf,
+                                               R* __restrict__ bp, V4<R>* __restrict__ bxf, const NeighborLists& NL) {
     using O = Ops<R, FAST>;
     __shared__ unsigned long long lds[2][8];
     __shared__ unsigned int s_list_waves;
-    if (NL.on) clear_table(NL.disp[(iter + 1) & 1], NL.buckets);  // for the next prediction
-    const BlockRange rg = B.blocks[blockIdx.x];
+    const BlockRange rg = B.blocks[range];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
     if (pressure_pass_skipped<R>(st, iter, P)) return;
@@ -1093,6 +1094,16 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint3
     unsigned long long* const dst[2] = {&st->max_dv[iter], &st->max_exp[iter]};
     block_max_to<2>(m, dst, lds);
     if (threadIdx.x == 0 && s_list_waves) atomicAdd(&st->list_waves, s_list_waves);
+}
+
+template <typename R, bool FAST>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
+                           const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
+                           const uint2* __restrict__ fcells, const V4<R>* __restrict__ xr,
+                           const V4<R>* __restrict__ xp, R* __restrict__ dv, R* __restrict__ p,
+                           V4<R>* __restrict__ xf, R* __restrict__ bp, V4<R>* __restrict__ bxf, NeighborLists NL) {
+    if (NL.on) clear_table(NL.disp[(iter + 1) & 1], NL.buckets);  // for the next prediction
+    pressure_range<R, FAST>(B, blockIdx.x, iter, bcells, bpos, binit, fcells, xr, xp, dv, p, xf, bp, bxf, NL);
 }
 
 // ------------------------------------------------------- LDS-staged pressure pass
@@ -1346,20 +1357,18 @@ __device__ void publish_iteration(const Batch<R>& B, uint32_t iter, HostMirror* 
 // prediction (pcisph.cl:175-193 into xp, Fluid.cpp:253-260) or, when this iteration
 // ends the scene's loop, the final in-place integration (Fluid.cpp:309-316) into
 // pos_out/vel_out.  Fluid ranges only.
+// One workgroup's fluid range `range` (k_pressure_force, k_correction_loop).
+// warm_pass: the pressure force of the network's initial pressures, before the first
+// prediction -- not an iteration of the loop.
 template <typename R, bool FAST>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
-                                 const V4<R>* __restrict__ bxf, const uint2* __restrict__ fcells,
-                                 const V4<R>* __restrict__ xf, const V4<R>* __restrict__ fo,
-                                 const V4<R>* __restrict__ vel, V4<R>* __restrict__ fp, V4<R>* __restrict__ xp,
-                                 V4<R>* __restrict__ pos_out, V4<R>* __restrict__ vel_out,
-                                 const R* __restrict__ p, HostMirror* host, SceneResult* res,
-                                 const unsigned int* tag, int warm_pass, V4<R>* __restrict__ xp_save,
-                                 NeighborLists NL) {
+__device__ __forceinline__ void pressure_force_range(
+    const Batch<R>& B, uint32_t range, uint32_t iter, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bxf,
+    const uint2* __restrict__ fcells, const V4<R>* __restrict__ xf, const V4<R>* __restrict__ fo,
+    const V4<R>* __restrict__ vel, V4<R>* __restrict__ fp, V4<R>* __restrict__ xp, V4<R>* __restrict__ pos_out,
+    V4<R>* __restrict__ vel_out, const R* __restrict__ p, int warm_pass, V4<R>* __restrict__ xp_save,
+    const NeighborLists& NL) {
     using O = Ops<R, FAST>;
-    // warm_pass: the pressure force of the network's initial pressures, before the
-    // first prediction -- not an iteration of the loop.
-    if (blockIdx.x == 0 && !warm_pass) publish_iteration<R>(B, iter, host, res, tag);
-    const BlockRange rg = B.blocks[blockIdx.x];
+    const BlockRange rg = B.blocks[range];
     const SphParams<R> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
     if (warm_pass ? !P.warm : iteration_skipped<R>(st, iter, P)) return;
@@ -1422,6 +1431,20 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B,
     }
     if (!last && !warm_pass && NL.on && P.disp_limit > 0.f)
         record_displacement(NL.disp[(iter + 1) & 1], bucket, moved);
+}
+
+template <typename R, bool FAST>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_force(Batch<R> B, uint32_t iter, const uint2* __restrict__ bcells,
+                                 const V4<R>* __restrict__ bxf, const uint2* __restrict__ fcells,
+                                 const V4<R>* __restrict__ xf, const V4<R>* __restrict__ fo,
+                                 const V4<R>* __restrict__ vel, V4<R>* __restrict__ fp, V4<R>* __restrict__ xp,
+                                 V4<R>* __restrict__ pos_out, V4<R>* __restrict__ vel_out,
+                                 const R* __restrict__ p, HostMirror* host, SceneResult* res,
+                                 const unsigned int* tag, int warm_pass, V4<R>* __restrict__ xp_save,
+                                 NeighborLists NL) {
+    if (blockIdx.x == 0 && !warm_pass) publish_iteration<R>(B, iter, host, res, tag);
+    pressure_force_range<R, FAST>(B, blockIdx.x, iter, bcells, bxf, fcells, xf, fo, vel, fp, xp, pos_out, vel_out, p,
+                                  warm_pass, xp_save, NL);
 }
 
 // The network's start: p = max(0, p_prev + y p_ref), y the inference kernel's output

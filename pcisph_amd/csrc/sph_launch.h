@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "sph_kernels.h"
+#include "sph_persistent.h"
 #include "sph_staged.h"
 
 namespace pcisph {
@@ -71,6 +72,10 @@ struct SphLaunch {
     static void pressure(const Launch& L, const DeviceArrays<R>& A, uint32_t iter);
     static void pressure_force(const Launch& L, const DeviceArrays<R>& A, uint32_t iter, HostMirror* host,
                                SceneResult* res, const unsigned int* tag, int warm_pass, V4<R>* xp_save);
+    // the whole correction loop as one cooperative launch (k_correction_loop, sph_persistent.h):
+    // the grid is the block table's range count, at most what the device holds resident
+    static hipError_t correction_loop(const Launch& L, const DeviceArrays<R>& A, uint32_t maxit, HostMirror* host,
+                                      SceneResult* res, const unsigned int* tag, const GridSync& g);
     // iteration 0 of the scenes a warm-started step starts from the network (k_pressure_dual,
     // k_select_start, k_adopt_plain_start); the pressure pass runs it for the others
     static void pressure_dual(const Launch& L, const DeviceArrays<R>& A, const DualArrays<R>& D);

@@ -79,6 +79,31 @@ void SphLaunch<R, FAST>::pressure_force(const Launch& L, const DeviceArrays<R>& 
 }
 
 template <typename R, bool FAST>
+hipError_t SphLaunch<R, FAST>::correction_loop(const Launch& L, const DeviceArrays<R>& A, uint32_t maxit,
+                                               HostMirror* host, SceneResult* res, const unsigned int* tag,
+                                               const GridSync& g) {
+    Batch<R> B = A.batch;
+    const uint32_t total = B.fluid_blocks + B.boundary_blocks;
+    if (!total || !maxit) return hipSuccess;
+    LoopArgs<R> a{A.bcells, A.bpos, A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp, A.bxf, A.fo, A.vel, A.fp,
+                  A.pos_in, A.vel_in, host, res, tag, A.NL, A.stage};
+    GridSync sync = g;
+    const void* kernel = (const void*)k_correction_loop<R, FAST, false>;
+    if constexpr (std::is_same<R, float>::value)
+        if (A.staged_pressure) kernel = (const void*)k_correction_loop<float, FAST, true>;
+    int device = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)L.block, 0);
+    if (e != hipSuccess) return e;
+    const uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 0) * (uint64_t)(cus > 0 ? cus : 0);
+    if (!resident) return hipErrorCooperativeLaunchTooLarge;
+    const uint32_t grid = (uint32_t)(total < resident ? total : resident);
+    void* args[] = {(void*)&B, (void*)&maxit, (void*)&a, (void*)&sync};
+    return hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(L.block), args, 0, L.stream);
+}
+
+template <typename R, bool FAST>
 void SphLaunch<R, FAST>::pressure_dual(const Launch& L, const DeviceArrays<R>& A, const DualArrays<R>& D) {
     const Batch<R>& B = A.batch;
     const uint32_t grid = B.fluid_blocks + B.boundary_blocks;

@@ -311,16 +311,15 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
 // the boundary work order (S.bwork), so a wavefront's 27-cell neighbourhoods overlap
 // instead of lying wherever the hash put their buckets.
 template <bool FAST>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
-    Batch<float> B, uint32_t iter, const uint2* __restrict__ bcells, const float4* __restrict__ bpos,
-    const float* __restrict__ binit, const uint2* __restrict__ fcells, const float4* __restrict__ xr,
-    const float4* __restrict__ xp, float* __restrict__ dv, float* __restrict__ p, float4* __restrict__ xf,
-    float* __restrict__ bp, float4* __restrict__ bxf, StageMaps S, NeighborLists NL) {
+__device__ __forceinline__ void pressure_staged_range(
+    const Batch<float>& B, uint32_t range, uint32_t iter, const uint2* __restrict__ bcells,
+    const float4* __restrict__ bpos, const float* __restrict__ binit, const uint2* __restrict__ fcells,
+    const float4* __restrict__ xr, const float4* __restrict__ xp, float* __restrict__ dv, float* __restrict__ p,
+    float4* __restrict__ xf, float* __restrict__ bp, float4* __restrict__ bxf, const StageMaps& S) {
     using O = Ops<float, FAST>;
     __shared__ float s_x[kStageCap], s_y[kStageCap], s_z[kStageCap];
     __shared__ unsigned long long lds[2][8];
-    if (NL.on) clear_table(NL.disp[(iter + 1) & 1], NL.buckets);  // for the next prediction
-    const BlockRange rg = B.blocks[blockIdx.x];
+    const BlockRange rg = B.blocks[range];
     const SphParams<float> P = B.scenes[rg.scene];
     LoopState* st = B.states + rg.scene;
     if (pressure_pass_skipped<float>(st, iter, P)) return;
@@ -448,6 +447,16 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
     }
     unsigned long long* const dst[2] = {&st->max_dv[iter], &st->max_exp[iter]};
     block_max_to<2>(m, dst, lds);
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_staged(
+    Batch<float> B, uint32_t iter, const uint2* __restrict__ bcells, const float4* __restrict__ bpos,
+    const float* __restrict__ binit, const uint2* __restrict__ fcells, const float4* __restrict__ xr,
+    const float4* __restrict__ xp, float* __restrict__ dv, float* __restrict__ p, float4* __restrict__ xf,
+    float* __restrict__ bp, float4* __restrict__ bxf, StageMaps S, NeighborLists NL) {
+    if (NL.on) clear_table(NL.disp[(iter + 1) & 1], NL.buckets);  // for the next prediction
+    pressure_staged_range<FAST>(B, blockIdx.x, iter, bcells, bpos, binit, fcells, xr, xp, dv, p, xf, bp, bxf, S);
 }
 
 }  // namespace pcisph
