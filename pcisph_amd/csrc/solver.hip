@@ -1061,14 +1061,15 @@ class Engine final : public EngineBase {
             rebuild_boxes(L, S);
             prepare_scenes(scenes, o);
         }
-        // Morton layout: bucket codes with `obits` bits per axis below the scene index
+        // Morton layout: bucket codes (order_code) with `obits` bits per axis and the
+        // outside-the-box bit below the scene index
         const uint32_t sbits = key_bits_for(S);
-        const uint32_t obits = std::min(7u, sbits < 32 ? (32u - sbits) / 3u : 0u);
+        const uint32_t obits = std::min(7u, sbits < 31 ? (31u - sbits) / 3u : 0u);
         // the layout pays for its second sort only where the data outgrows the caches:
         // measured 15% slower at 64k fluid particles, 3% faster at 256k, 20% at 1M
         const bool layout =
             (o.work_order & 1) && obits >= 2 && A_.n > 1 && (A_.n >= kLayoutMinParticles || (o.work_order & 8));
-        const uint32_t layout_bits = layout ? 3 * obits + sbits : 0;
+        const uint32_t layout_bits = layout ? 3 * obits + 1 + sbits : 0;
         if (!layout) canonicalize();
         if (layout) ensure_order_buffers();
         A_.batch = Batch<R>{scene_table_, block_table_, states_, S, fluid_blocks_, boundary_blocks_,
@@ -1083,7 +1084,7 @@ class Engine final : public EngineBase {
             all_buckets = all_buckets && (!A_.NL.on || !(scene_host_[s].disp_limit > 0.f) || !n_[s]);
         const bool staged = std::is_same<R, float>::value && A_.n > 0 && !(o.work_order & 32) &&
                             ((o.work_order & 16) || all_buckets);
-        const uint32_t stage_bits = 3 * obits + sbits;
+        const uint32_t stage_bits = 3 * obits + 1 + sbits;
         A_.staged_pressure = staged;
         if (staged) {
             ensure_stage_buffers(fluid_blocks_);

@@ -96,12 +96,6 @@ __global__ void k_translate(V4<R>* __restrict__ p, uint32_t n, R dx, R dy, R dz)
     p[i] = v;
 }
 
-// The low `bits` bits of v spread to every third bit (bit k -> bit 3k).
-__device__ __forceinline__ uint32_t spread3(uint32_t v, uint32_t bits) {
-    uint32_t out = 0;
-    for (uint32_t k = 0; k < bits; ++k) out |= ((v >> k) & 1u) << (3 * k);
-    return out;
-}
 
 // The Morton layout of the fluid particles.  The reference's arrays are in bucket
 // order (sort_utils.cl:25-78); this build stores each bucket's particles contiguously
@@ -127,9 +121,8 @@ __global__ void k_layout_codes(const SphParams<R>* __restrict__ scenes, const Bl
     srcc[c] = canon[s];
     const uint32_t k = K[c];
     if (c == 0 || K[c - 1] != k) {
-        const int3 q = cell_of4<R>(pos[s], scenes[rg.scene].cell);
-        code[k] = (rg.scene << (3 * bits)) | spread3((uint32_t)q.x, bits) | (spread3((uint32_t)q.y, bits) << 1) |
-                  (spread3((uint32_t)q.z, bits) << 2);
+        const SphParams<R>& P = scenes[rg.scene];
+        code[k] = order_code<R>(P, rg.scene, cell_of4<R>(pos[s], P.cell), bits);
     }
 }
 

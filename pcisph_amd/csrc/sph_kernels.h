@@ -79,6 +79,13 @@ struct SphParams {
                                      // step, the scene not left out of it); 0 in every other step
 };
 
+// The low `bits` bits of v spread to every third bit (bit k -> bit 3k).
+__device__ __forceinline__ uint32_t spread_bits3(uint32_t v, uint32_t bits) {
+    uint32_t out = 0;
+    for (uint32_t k = 0; k < bits; ++k) out |= ((v >> k) & 1u) << (3 * k);
+    return out;
+}
+
 // One workgroup's share of a launch: particles [begin, end) of one scene.
 enum : uint32_t { kRangeBoundary = 1u, kRangeFirstOfScene = 2u };
 struct BlockRange {
@@ -166,6 +173,25 @@ struct Batch {
     const uint2* fgrid;           // dense range grids of the scenes' cell boxes (fluid, boundary), or null
     const uint2* bgrid;
 };
+
+// Code of a cell for the storage and work orders: the Morton interleave of `bits`
+// bits per axis of its coordinates -- relative to the scene's cell box when it has
+// one -- with the scene index above.  Relative to the box, cells outside its first
+// 2^bits cells along some axis (particles thrown clear of the scene) get the bit above
+// the Morton code, so they follow every cell inside instead of wrapping round onto
+// it; without a box the coordinates wrap modulo 2^bits.  3 bits + 1 + scene bits.
+template <typename R>
+__device__ __forceinline__ uint32_t order_code(const SphParams<R>& P, uint32_t scene, int3 q, uint32_t bits) {
+    uint32_t x = (uint32_t)q.x, y = (uint32_t)q.y, z = (uint32_t)q.z, out = 0;
+    if (P.box_dim[0]) {
+        x -= P.box_lo[0];
+        y -= P.box_lo[1];
+        z -= P.box_lo[2];
+        out = ((x | y | z) >> bits) ? 1u : 0u;
+    }
+    return (scene << (3 * bits + 1)) | (out << (3 * bits)) | spread_bits3(x, bits) | (spread_bits3(y, bits) << 1) |
+           (spread_bits3(z, bits) << 2);
+}
 
 // A scene's fluid (or boundary) bucket ranges: the batch's bucket table and grid.
 template <typename R>

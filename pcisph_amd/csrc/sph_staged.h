@@ -94,14 +94,7 @@ __device__ __forceinline__ uint2 block_scan2(uint32_t a, uint32_t b, uint2& tota
     return make_uint2(ba + xa - a, bb + xb - b);
 }
 
-// Code of a particle's own cell: scene index above the Morton code of the cell
-// coordinates modulo 2^bits (the Morton layout's code of buckets, taken per particle).
-__device__ __forceinline__ uint32_t stage_spread3(uint32_t v, uint32_t bits) {
-    uint32_t out = 0;
-    for (uint32_t k = 0; k < bits; ++k) out |= ((v >> k) & 1u) << (3 * k);
-    return out;
-}
-
+// Work-order keys: the order code (order_code) of each particle's own cell.
 template <typename R>
 __global__ void k_stage_codes(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
                               const V4<R>* __restrict__ pos, uint32_t bits, uint32_t* __restrict__ keys,
@@ -109,9 +102,8 @@ __global__ void k_stage_codes(const SphParams<R>* __restrict__ scenes, const Blo
     const BlockRange rg = blocks[blockIdx.x];
     const uint32_t i = rg.begin + threadIdx.x;
     if (i >= rg.end) return;
-    const int3 q = cell_of4<R>(pos[i], scenes[rg.scene].cell);
-    keys[i] = (rg.scene << (3 * bits)) | stage_spread3((uint32_t)q.x, bits) |
-              (stage_spread3((uint32_t)q.y, bits) << 1) | (stage_spread3((uint32_t)q.z, bits) << 2);
+    const SphParams<R>& P = scenes[rg.scene];
+    keys[i] = order_code<R>(P, rg.scene, cell_of4<R>(pos[i], P.cell), bits);
     vals[i] = i;
 }
 
