@@ -357,7 +357,9 @@ int pcisph_translate_boundary(pcisph_solver* s, float dx, float dy, float dz);
 int pcisph_translate_boundaries(pcisph_solver* s, const float* shifts, size_t count);
 
 /* Autotuner: times `trial_steps` steps of the current scene(s) for each threads-per-block
- * choice (64..512) and radix digit width (5..8 bits) from the same saved starting state,
+ * choice (64..512), radix digit width (5..8 bits) and tiling of the pressure pass
+ * (work_order bits 4/5: LDS-staged workgroup tiles when no lists serve it, always, or
+ * never) from the same saved starting state,
  * keeps the fastest in the solver options (and returns them in `chosen` if non-NULL),
  * and restores the state (particles, pending boundary sorts, warm-start eligibility,
  * sample collection; the options too if it fails).  It takes the snapshot slot of

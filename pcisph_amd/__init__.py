@@ -233,10 +233,11 @@ class Fluid(SolverBase):
         return loss.value
 
     def autotune(self, trial_steps: int = 3) -> dict:
-        """Pick block size and radix digit width by timing trial steps (state unchanged)."""
+        """Pick block size, radix digit width and the pressure pass's tiling (work_order
+        bits 4/5) by timing trial steps (state unchanged)."""
         o = SolverOptions()
         self._call("autotune", C.c_uint32(trial_steps), C.byref(o))
-        return {"block_size": o.block_size, "sort_digit_bits": o.sort_digit_bits}
+        return {"block_size": o.block_size, "sort_digit_bits": o.sort_digit_bits, "work_order": o.work_order}
 
     def translate_boundary(self, dx: float, dy: float, dz: float):
         """Move the walls rigidly by (dx, dy, dz); the boundary is re-sorted next step."""

@@ -2345,9 +2345,9 @@ int pcisph_set_warm_start(pcisph_solver* s, int enable) {
 
 // ------------------------------------------------------------------ autotuner
 // Times `trial_steps` steps of the current scene for each block size of the particle
-// kernels and each radix digit width, from the same starting state each time (a
-// device snapshot restored after every trial), then keeps the fastest pair in the
-// solver options.  The state after the call is the state before it: particle state,
+// kernels, then each radix digit width, then each tiling of the pressure pass
+// (work_order bits 4 and 5), from the same starting state each time (a device
+// snapshot restored after every trial), and keeps the fastest options.  The state after the call is the state before it: particle state,
 // pending boundary sorts, each scene's warm-start eligibility and
 // sample collection; on an error the options are the caller's again too.
 int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_options* chosen) {
@@ -2389,10 +2389,11 @@ int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_option
         }};
         double best = 1e300;
         pcisph_solver_options best_opt = base;
-        auto trial = [&](uint32_t block, uint32_t digit) {
+        auto trial = [&](uint32_t block, uint32_t digit, uint32_t work_order) {
             s->opt = base;
             s->opt.block_size = block;
             s->opt.sort_digit_bits = digit;
+            s->opt.work_order = work_order;
             s->eng->restore();
             reset_boundary();
             step_impl(s, nullptr);  // warm-up: caches, code objects
@@ -2408,9 +2409,15 @@ int pcisph_autotune(pcisph_solver* s, uint32_t trial_steps, pcisph_solver_option
                 best_opt = s->opt;
             }
         };
-        for (uint32_t block : {64u, 128u, 256u, 512u}) trial(block, 8u);
+        const uint32_t wo = base.work_order & ~48u;
+        for (uint32_t block : {64u, 128u, 256u, 512u}) trial(block, 8u, base.work_order);
         const uint32_t block = best_opt.block_size;
-        for (uint32_t digit : {5u, 6u, 7u}) trial(block, digit);
+        for (uint32_t digit : {5u, 6u, 7u}) trial(block, digit, base.work_order);
+        // the pressure pass's tiles: LDS-staged workgroup tiles where no lists serve it
+        // (the default), always (bit 4), or never (bit 5: every lane walks its buckets)
+        const uint32_t digit = best_opt.sort_digit_bits;
+        for (uint32_t tiles : {0u, 16u, 32u})
+            if ((wo | tiles) != base.work_order) trial(block, digit, wo | tiles);
         guard.fn();
         guard.fn = [] {};
         s->opt = best_opt;

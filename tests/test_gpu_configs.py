@@ -305,6 +305,7 @@ def test_autotune_keeps_state_and_parity(gpu, oracle_lib):
     chosen = g.autotune(trial_steps=2)
     print(f"  chosen {chosen}", flush=True)
     assert chosen["block_size"] in (64, 128, 256, 512) and chosen["sort_digit_bits"] in (5, 6, 7, 8)
+    assert chosen["work_order"] & ~48 == 3 and chosen["work_order"] & 48 in (0, 16, 32)
     so += [o.step() for _ in range(2)]
     sg += [g.step() for _ in range(2)]
     assert_same_stats(sg, so, "autotune")
@@ -333,7 +334,7 @@ def test_autotune_restores_warm_start_eligibility(gpu):
     b = make()
     sb = [b.step() for _ in range(2)]
     b.set_solver_options(min_iterations=2, max_iterations=2, block_size=chosen["block_size"],
-                         sort_digit_bits=chosen["sort_digit_bits"])
+                         sort_digit_bits=chosen["sort_digit_bits"], work_order=chosen["work_order"])
     sb += [b.step() for _ in range(3)]
     assert [s["converged"] for s in sa[:2]] == [0, 0]
     for x, y in zip(sa, sb):
