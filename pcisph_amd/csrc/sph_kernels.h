@@ -1414,13 +1414,27 @@ __device__ __forceinline__ void pressure_force_range(
             R dx, dy, dz;
             const R r2 = dist2<R>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
             if (r2 > P.h2) return;
-            const R r = O::sqrt(r2);
-            if (r > P.h || r < R(0.0001f)) return;  // kernel_spiky_d1 (pcisph.cl:28-37)
-            const R dr = P.h - r;
-            const R factor = sf + o.w;
-            fx += O::div(dx * dr * dr, r) * factor;
-            fy += O::div(dy * dr * dr, r) * factor;
-            fz += O::div(dz * dr * dr, r) * factor;
+            if constexpr (FAST) {
+                // one reciprocal square root for r and 1/r: the same terms to within the
+                // fast mode's rounding (its 0.0001 cut taken on r^2)
+                if (r2 < 1e-8f) return;  // kernel_spiky_d1 (pcisph.cl:28-37)
+                const float q = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * q;
+                if (r > P.h) return;
+                const float dr = P.h - r;
+                const float sc = dr * dr * q * (sf + o.w);
+                fx += dx * sc;
+                fy += dy * sc;
+                fz += dz * sc;
+            } else {
+                const R r = O::sqrt(r2);
+                if (r > P.h || r < R(0.0001f)) return;  // kernel_spiky_d1 (pcisph.cl:28-37)
+                const R dr = P.h - r;
+                const R factor = sf + o.w;
+                fx += O::div(dx * dr * dr, r) * factor;
+                fy += O::div(dy * dr * dr, r) * factor;
+                fz += O::div(dz * dr * dr, r) * factor;
+            }
         };
         // positions here are x, where the lists are exact
         const bool lists = NL.on != 0;
