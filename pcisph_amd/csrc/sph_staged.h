@@ -295,6 +295,26 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
     }
 }
 
+// The poly6 terms of one staged row, LDS candidates [a, a + n) in order (x and y in
+// `xy`, z in `z`), added to `pd`.  Four candidates per step from two LDS pointers:
+// paired loads and no per-candidate address arithmetic.
+template <bool FAST>
+__device__ __forceinline__ float staged_row(const float2* __restrict__ xy, const float* __restrict__ z, uint32_t a,
+                                            uint32_t n, const float4& pr, float h2, float pd) {
+    const float2* q = xy + a;
+    const float* qz = z + a;
+    const float2* const end = q + n;
+    for (; q + 4 <= end; q += 4, qz += 4) {
+        float4 o[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) o[u] = make_float4(q[u].x, q[u].y, qz[u], 0.f);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pd = poly6_add<float, FAST>(pd, pr, o[u], h2);
+    }
+    for (; q < end; ++q, ++qz) pd = poly6_add<float, FAST>(pd, pr, make_float4(q->x, q->y, *qz, 0.f), h2);
+    return pd;
+}
+
 // update_pressure (pcisph.cl:212-263), the fluid ranges with their candidates from the
 // workgroup's LDS staging (see above) -- the same results as k_pressure's bucket walk --
 // and the boundary ranges (boundary mode) walking their fluid buckets in global
@@ -309,7 +329,8 @@ __device__ __forceinline__ void pressure_staged_range(
     const float4* __restrict__ xr, const float4* __restrict__ xp, float* __restrict__ dv, float* __restrict__ p,
     float4* __restrict__ xf, float* __restrict__ bp, float4* __restrict__ bxf, const StageMaps& S) {
     using O = Ops<float, FAST>;
-    __shared__ float s_x[kStageCap], s_y[kStageCap], s_z[kStageCap];
+    __shared__ float2 s_xy[kStageCap];
+    __shared__ float s_z[kStageCap];
     __shared__ unsigned long long lds[2][8];
     const BlockRange rg = B.blocks[range];
     const SphParams<float> P = B.scenes[rg.scene];
@@ -325,11 +346,7 @@ __device__ __forceinline__ void pressure_staged_range(
             float pd = 0.f;
             pd += binit[b];
             walk_buckets<float>(cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xp[j]; },
-                                [&](const float4& o, uint32_t) {
-                                    float dx, dy, dz;
-                                    pd += poly6<float>(dist2<float>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz),
-                                                       P.h2);
-                                });
+                                [&](const float4& o, uint32_t) { pd = poly6_add<float, FAST>(pd, self, o, P.h2); });
             pd *= P.density_scale;
             const float v = fmaxf(0.f, pd - P.rho0);
             if (v != 0.f) {
@@ -368,8 +385,7 @@ __device__ __forceinline__ void pressure_staged_range(
             for (int u = 0; u < kCopyBatch; ++u) {
                 const uint32_t q = q0 + u * blockDim.x;
                 if (q < all) {
-                    s_x[q] = v[u].x;
-                    s_y[q] = v[u].y;
+                    s_xy[q] = make_float2(v[u].x, v[u].y);
                     s_z[q] = v[u].z;
                 }
             }
@@ -383,10 +399,7 @@ __device__ __forceinline__ void pressure_staged_range(
         const float4 self = xr[i], pr = xp[i];
         const int3 c = cell_of4<float>(self, P.cell);
         float pd = 0.f;
-        auto add = [&](const float4& o, uint32_t) {
-            float dx, dy, dz;
-            pd += poly6<float>(dist2<float>(pr.x, pr.y, pr.z, o.x, o.y, o.z, dx, dy, dz), P.h2);
-        };
+        auto add = [&](const float4& o, uint32_t) { pd = poly6_add<float, FAST>(pd, pr, o, P.h2); };
         if (P.nb) {
             CellTable BT = cell_table(P, bcells, B.bgrid);
             BT.prune = far_prunable<float>(displacement<float>(pr, self), (float)P.h, P.cell);
@@ -399,11 +412,8 @@ __device__ __forceinline__ void pressure_staged_range(
 #pragma unroll
                     for (int row = 0; row < 9; ++row) d[row] = S.brows[(size_t)row * B.n_total + k];
 #pragma unroll 1
-                    for (int row = 0; row < 9; ++row) {
-                        const uint32_t a = d[row] & 0xFFFFu;
-                        gather_range(a, a + (d[row] >> 16),
-                                     [&](uint32_t j) { return make_float4(s_x[j], s_y[j], s_z[j], 0.f); }, add);
-                    }
+                    for (int row = 0; row < 9; ++row)
+                        pd = staged_row<FAST>(s_xy, s_z, d[row] & 0xFFFFu, d[row] >> 16, pr, P.h2, pd);
                 }
             } else {
                 walk_buckets<float>(BT, c, [&](uint32_t j) { return bpos[j]; }, add);
@@ -414,11 +424,8 @@ __device__ __forceinline__ void pressure_staged_range(
 #pragma unroll
             for (int row = 0; row < 9; ++row) d[row] = S.rows[(size_t)row * B.n_total + k];
 #pragma unroll 1
-            for (int row = 0; row < 9; ++row) {
-                const uint32_t a = d[row] & 0xFFFFu;
-                gather_range(a, a + (d[row] >> 16), [&](uint32_t j) { return make_float4(s_x[j], s_y[j], s_z[j], 0.f); },
-                             add);
-            }
+            for (int row = 0; row < 9; ++row)
+                pd = staged_row<FAST>(s_xy, s_z, d[row] & 0xFFFFu, d[row] >> 16, pr, P.h2, pd);
         } else {
             walk_buckets<float>(cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xp[j]; }, add);
         }
