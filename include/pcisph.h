@@ -141,7 +141,12 @@ typedef struct pcisph_solver_options {
                                        the passes (identical results; not in steps the stage
                                        profiler times, warm-started steps or with bit 2).
                                        Default 3 (bits 0 and 1). */
-    uint32_t reserved;
+    uint32_t stage_cap;             /* LDS-staged pressure pass (work_order bits 4/5): fluid
+                                       candidates a workgroup stages, 64..2048; 0 = 2048 (the
+                                       kernel's LDS).  A workgroup with more, up to twice this,
+                                       stages the rows that fit and walks its other rows'
+                                       buckets in global memory; one with more still walks
+                                       every bucket there.  Results are identical. */
 } pcisph_solver_options;
 
 /* Per-step report (the reference only has a compiled-out timer, Fluid.cpp:319-326). */

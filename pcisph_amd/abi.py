@@ -54,7 +54,7 @@ class SolverOptions(C.Structure):
         ("neighbor_skin", C.c_float),
         ("graph_iterations", C.c_uint32),
         ("work_order", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("stage_cap", C.c_uint32),
     ]
 
 
@@ -182,9 +182,9 @@ class SolverBase:
 
     def set_solver_options(self, min_iterations=3, max_iterations=7, launch_boundary_init=0,
                            resort_boundary=0, block_size=0, sort_digit_bits=0, neighbor_skin=0.2,
-                           graph_iterations=4, work_order=3):
+                           graph_iterations=4, work_order=3, stage_cap=0):
         o = SolverOptions(min_iterations, max_iterations, launch_boundary_init, resort_boundary,
-                          block_size, sort_digit_bits, neighbor_skin, graph_iterations, work_order)
+                          block_size, sort_digit_bits, neighbor_skin, graph_iterations, work_order, stage_cap)
         self._call("set_solver_options", C.byref(o))
 
     def field_size(self, field: int) -> int:

@@ -1015,6 +1015,7 @@ class Engine final : public EngineBase {
             put(chunk);
         put(staged), ptr(A_.stage.work), ptr(A_.stage.rows), ptr(A_.stage.src), ptr(A_.stage.info);
         ptr(A_.stage.bwork), ptr(A_.stage.brows), ptr(A_.stage.binfo);
+        put(A_.stage.cap);
         put(layout_bits), put(morton_bits);
         put(A_.tile_pressure);
         put(A_.n), put(A_.nb), put(A_.B);
@@ -1094,7 +1095,8 @@ class Engine final : public EngineBase {
                                  stage_info_,
                                  stage_brows_,
                                  stage_binfo_,
-                                 bstage_vals_[radix_sort_result(A_.nb, stage_bits, digit)]};
+                                 bstage_vals_[radix_sort_result(A_.nb, stage_bits, digit)],
+                                 o.stage_cap ? o.stage_cap : kStageCap};
         }
         if (features) ensure_warm_buffers(warm);
         if (warm) upload_warm_rows(scenes);
@@ -1882,6 +1884,8 @@ int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt
         if (!(opt->neighbor_skin >= 0.f && opt->neighbor_skin <= 2.f))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "neighbor_skin must be in [0, 2]");
         if (opt->work_order > 127) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..127");
+        if (opt->stage_cap && (opt->stage_cap < 64 || opt->stage_cap > kStageCap))
+            throw Error(PCISPH_ERR_INVALID_ARGUMENT, "stage_cap must be 0 or in 64..2048");
         s->opt = *opt;
     });
 }

@@ -29,7 +29,10 @@
 // the same launch (boundary mode) take their particles in the Morton order of their
 // own cells (the boundary work order, sorted when the boundary is).  A workgroup whose
 // staging does not fit (kStageCap candidates, kStageMaxCells cells, kStageLines lines,
-// cell coordinates of magnitude 2^29 or more) walks global memory instead.
+// cell coordinates of magnitude 2^29 or more) walks global memory instead; one whose
+// candidates exceed the staging cap (StageMaps::cap, at most kStageCap) but not twice
+// it stages the lines that fit, and each particle walks a row lying wholly in LDS
+// there and any other row's three buckets in global memory, in the same order.
 #pragma once
 
 #include "sph_kernels.h"
@@ -37,7 +40,7 @@
 namespace pcisph {
 
 constexpr uint32_t kRangeTileShift = 8;      // fluid ranges: their workgroup's staging index above the flag bits
-constexpr uint32_t kStageCap = 2048;         // staged candidates per workgroup (x, y, z: 24 KB of LDS)
+constexpr uint32_t kStageCap = 2048;         // staged candidates per workgroup at most (x, y, z: 24 KB of LDS)
 constexpr uint32_t kStageMaxCells = 2048;    // staged cells per workgroup
 constexpr uint32_t kStageLines = 1024;       // (y, z) lines of a workgroup's box
 constexpr uint32_t kStageFallback = 0xFFFFFFFFu;
@@ -49,10 +52,11 @@ struct StageMaps {
     const uint32_t* work;  // [n_total] work index -> fluid slot (per-step Morton order of own cells)
     uint32_t* rows;        // [9][n_total] per work index: row r's LDS start | length << 16
     uint32_t* src;         // [workgroups][kStageCap] slot of each staged candidate (fluid, then boundary)
-    uint32_t* info;        // [workgroups] staged fluid candidate count, or kStageFallback
+    uint32_t* info;        // [workgroups] fluid candidate count (the first kStageCap staged), or kStageFallback
     uint32_t* brows;       // [9][n_total] per work index: boundary row r's LDS start | length << 16
     uint32_t* binfo;       // [workgroups] staged boundary candidate count, or kStageFallback
     const uint32_t* bwork; // [nb_total] boundary work index -> boundary slot (Morton order of own cells)
+    uint32_t cap;          // candidates staged per workgroup, 64..kStageCap; up to 2 cap partly staged
 };
 
 // The bucket range of cell (x, y, z) as for_each_range_row reads it (grid flags
@@ -257,12 +261,12 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
     }
     const uint32_t total = tot.x;
     // the boundary candidates follow the fluid ones in the staging, if both fit
-    const uint32_t btotal = total + tot.y <= kStageCap ? tot.y : kStageFallback;
+    const uint32_t btotal = total + tot.y <= S.cap ? tot.y : kStageFallback;
     if (threadIdx.x == 0) {
         s_off[ncells] = total;
         s_boff[ncells] = tot.y;
     }
-    if (total > kStageCap) {
+    if (total > 2u * S.cap) {
         if (threadIdx.x == 0) S.info[tile] = S.binfo[tile] = kStageFallback;
         return;
     }
@@ -271,7 +275,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
     uint32_t* src = S.src + (size_t)tile * kStageCap;
     for (uint32_t q = threadIdx.x; q < ncells; q += blockDim.x) {
         const uint32_t o = s_off[q], n = s_off[q + 1] - o, b = s_start[q];
-        for (uint32_t e = 0; e < n; ++e) src[o + e] = b + e;
+        const uint32_t fit = o < S.cap ? min(n, S.cap - o) : 0u;  // past the cap: not staged
+        for (uint32_t e = 0; e < fit; ++e) src[o + e] = b + e;
         if (btotal != kStageFallback) {
             const uint32_t bo = total + s_boff[q], bn = s_boff[q + 1] - s_boff[q], bb = s_bstart[q];
             for (uint32_t e = 0; e < bn; ++e) src[bo + e] = bb + e;
@@ -312,6 +317,20 @@ __device__ __forceinline__ float staged_row(const float2* __restrict__ xy, const
         for (int u = 0; u < 4; ++u) pd = poly6_add<float, FAST>(pd, pr, o[u], h2);
     }
     for (; q < end; ++q, ++qz) pd = poly6_add<float, FAST>(pd, pr, make_float4(q->x, q->y, *qz, 0.f), h2);
+    return pd;
+}
+
+// The same row of a partly staged workgroup that lies past the staging: row `row` of
+// cell c's 27 (dz-major, then dy) with its three bucket ranges read from the cell
+// table and their candidates' predicted positions from global memory.
+template <bool FAST>
+__device__ __forceinline__ float global_row(const CellTable& T, int3 c, int row, const float4* __restrict__ xp,
+                                            const float4& pr, float h2, float pd) {
+    const uint32_t x = (uint32_t)c.x - 1u, y = (uint32_t)c.y + (uint32_t)(row % 3 - 1),
+                   z = (uint32_t)c.z + (uint32_t)(row / 3 - 1);
+    gather_row<kGatherBatch>(cell_range(T, x, y, z), cell_range(T, x + 1u, y, z), cell_range(T, x + 2u, y, z),
+                             [&](uint32_t j) { return xp[j]; },
+                             [&](const float4& o, uint32_t) { pd = poly6_add<float, FAST>(pd, pr, o, h2); });
     return pd;
 }
 
@@ -364,10 +383,12 @@ __device__ __forceinline__ void pressure_staged_range(
     const bool staged = total != kStageFallback;
     const uint32_t btotal = staged ? S.binfo[tile] : kStageFallback;
     const bool bstaged = btotal != kStageFallback;
+    const uint32_t fitted = staged ? min(total, S.cap) : 0u;  // fluid candidates in LDS
     if (staged) {
         // kCopyBatch independent loads in flight per thread (indices, then positions):
         // the fluid candidates' predicted positions, then the boundary candidates'
-        const uint32_t all = total + (bstaged ? btotal : 0u);
+        // (staged only when every fluid candidate is)
+        const uint32_t all = fitted + (bstaged ? btotal : 0u);
         const uint32_t* src = S.src + (size_t)tile * kStageCap;
         for (uint32_t q0 = threadIdx.x; q0 < all; q0 += kCopyBatch * blockDim.x) {
             uint32_t j[kCopyBatch];
@@ -377,7 +398,7 @@ __device__ __forceinline__ void pressure_staged_range(
             for (int u = 0; u < kCopyBatch; ++u) {
                 const uint32_t q = q0 + u * blockDim.x, qq = q < all ? q : q0;
                 j[u] = src[qq];
-                from[u] = qq < total ? xp : bpos;
+                from[u] = qq < fitted ? xp : bpos;
             }
 #pragma unroll
             for (int u = 0; u < kCopyBatch; ++u) v[u] = from[u][j[u]];
@@ -423,9 +444,19 @@ __device__ __forceinline__ void pressure_staged_range(
             uint32_t d[9];
 #pragma unroll
             for (int row = 0; row < 9; ++row) d[row] = S.rows[(size_t)row * B.n_total + k];
+            if (total <= S.cap) {
 #pragma unroll 1
-            for (int row = 0; row < 9; ++row)
-                pd = staged_row<FAST>(s_xy, s_z, d[row] & 0xFFFFu, d[row] >> 16, pr, P.h2, pd);
+                for (int row = 0; row < 9; ++row)
+                    pd = staged_row<FAST>(s_xy, s_z, d[row] & 0xFFFFu, d[row] >> 16, pr, P.h2, pd);
+            } else {
+                const CellTable FT = cell_table(P, fcells, B.fgrid);
+#pragma unroll 1
+                for (int row = 0; row < 9; ++row) {
+                    const uint32_t a = d[row] & 0xFFFFu, n = d[row] >> 16;
+                    if (a + n <= S.cap) pd = staged_row<FAST>(s_xy, s_z, a, n, pr, P.h2, pd);
+                    else pd = global_row<FAST>(FT, c, row, xp, pr, P.h2, pd);
+                }
+            }
         } else {
             walk_buckets<float>(cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xp[j]; }, add);
         }

@@ -374,6 +374,35 @@ def test_staged_pressure_block_sizes(gpu, oracle_lib, block):
     assert_same_state(g, o, tag=f"staged block {block}")
 
 
+@pytest.mark.parametrize("cap", [256, 512, 1024])
+def test_staged_pressure_partial_bit_exact(gpu, oracle_lib, cap):
+    """Staging caps below the workgroups' candidate counts (~450-600 per 64-particle
+    workgroup here): a workgroup with up to twice the cap stages the rows that fit and
+    walks its other rows' buckets in global memory, one with more walks global memory
+    throughout -- the oracle's bits either way."""
+    scene = small_scenes()[1]
+    print(f"case staged pressure partial cap {cap}", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 2, max_iterations=20)
+    sg = run(g, scene, 2, max_iterations=20, work_order=STAGED, stage_cap=cap)
+    assert_same_stats(sg, so, f"staged cap {cap}")
+    assert_same_state(g, o, tag=f"staged cap {cap}")
+
+
+def test_staged_pressure_partial_fast_equals_full(gpu):
+    """Fast FP32: rows walked in global memory add the same terms in the same order as
+    rows walked in LDS, so a partly staged pass equals the fully staged one bit for bit."""
+    scene = small_scenes()[0]
+    print("case staged pressure partial fast", flush=True)
+    runs = []
+    for cap in (0, 256):
+        g = gpu.Fluid(precision=gpu.PRECISION_FAST)
+        runs.append((g, run(g, scene, 3, max_iterations=20, work_order=STAGED, stage_cap=cap)))
+    assert_same_stats(runs[1][1], runs[0][1], "fast cap 256")
+    assert_same_state(runs[1][0], runs[0][0], tag="fast cap 256")
+
+
 def test_1m_staged_two_steps_bit_exact(gpu, oracle_1m):
     scene, ref = oracle_1m
     print("case 1m staged pressure ieee, 2 steps", flush=True)
@@ -385,6 +414,22 @@ def test_1m_staged_two_steps_bit_exact(gpu, oracle_1m):
         snap_g = snapshot(g)
         for name in snap_o:
             np.testing.assert_array_equal(snap_g[name], snap_o[name], err_msg=f"staged step {k + 1} {name}")
+    g.close()
+
+
+def test_1m_partial_staging_two_steps_bit_exact(gpu, oracle_1m):
+    """The 1M dam break with a staging cap of 1,024: its ~1,800-candidate workgroups are
+    staged in part (rows past the cap walked in global memory)."""
+    scene, ref = oracle_1m
+    print("case 1m partly staged pressure ieee, 2 steps", flush=True)
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    g.load_scene(scene, min_iterations=3, max_iterations=3, work_order=STAGED, stage_cap=1024)
+    for k, (st_o, snap_o) in enumerate(ref):
+        st_g = g.step()
+        assert_same_stats([st_g], [st_o], f"partly staged step {k + 1}")
+        snap_g = snapshot(g)
+        for name in snap_o:
+            np.testing.assert_array_equal(snap_g[name], snap_o[name], err_msg=f"partly staged step {k + 1} {name}")
     g.close()
 
 
