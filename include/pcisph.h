@@ -139,8 +139,13 @@ typedef struct pcisph_solver_options {
                                        bit 6: the correction loop runs as one cooperative
                                        launch whose workgroups meet at a grid barrier between
                                        the passes (identical results; not in steps the stage
-                                       profiler times, warm-started steps or with bit 2).
-                                       Default 3 (bits 0 and 1). */
+                                       profiler times, warm-started steps or with bit 2);
+                                       bit 7: the step's radix sorts run the one-sweep variant
+                                       (one read of the keys for every digit histogram, then
+                                       one scatter kernel per digit whose tile offsets come
+                                       from a decoupled look-back; identical results; a look-
+                                       back wait past 2 s fails the step with
+                                       PCISPH_ERR_DEVICE).  Default 3 (bits 0 and 1). */
     uint32_t stage_cap;             /* LDS-staged pressure pass (work_order bits 4/5): fluid
                                        candidates a workgroup stages, 64..2048; 0 = 2048 (the
                                        kernel's LDS).  A workgroup with more, up to twice this,

@@ -30,6 +30,7 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-Wall",
 UNITS = {
     "solver.hip": ["-ffp-contract=off"],
     "radix_sort.hip": ["-ffp-contract=off"],
+    "radix_onesweep.hip": ["-ffp-contract=off"],
     "sph_ieee.hip": ["-ffp-contract=off"],
     "sph_fast.hip": ["-ffp-contract=fast"],
     "mlp.hip": ["-ffp-contract=fast"],

@@ -213,7 +213,9 @@ void exclusive_scan(const Launch& L, uint32_t* data, uint32_t count, uint32_t* p
 size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits) {
     const size_t tiles = (n + kTile - 1) / kTile;
     const size_t words = (size_t(1) << digit_bits) * (tiles ? tiles : 1);
-    return words + scan_partial_words(words);  // histograms, then the scan's chunk totals
+    const size_t own = words + scan_partial_words(words);  // histograms, then the scan's chunk totals
+    const size_t one = radix_onesweep_scratch_words(n, digit_bits);
+    return own > one ? own : one;
 }
 
 int radix_sort_result(uint32_t n, uint32_t key_bits, uint32_t digit_bits) {
@@ -223,6 +225,7 @@ int radix_sort_result(uint32_t n, uint32_t key_bits, uint32_t digit_bits) {
 
 int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
                uint32_t digit_bits, uint32_t* scratch) {
+    if (L.onesweep) return radix_sort_onesweep(L, keys, vals, n, key_bits, digit_bits, scratch);
     if (n <= 1 || key_bits == 0) return 0;
     const uint32_t tiles = (n + kTile - 1) / kTile;
     int cur = 0;

@@ -1011,7 +1011,7 @@ class Engine final : public EngineBase {
         std::vector<uint64_t> k;
         auto put = [&](uint64_t v) { k.push_back(v); };
         auto ptr = [&](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
-        put(L.block), put(fast), put(warm), put(features), put(all_features), put(digit), put(key_bits), put(maxit),
+        put(L.block), put(L.onesweep), put(fast), put(warm), put(features), put(all_features), put(digit), put(key_bits), put(maxit),
             put(chunk);
         put(staged), ptr(A_.stage.work), ptr(A_.stage.rows), ptr(A_.stage.src), ptr(A_.stage.info);
         ptr(A_.stage.bwork), ptr(A_.stage.brows), ptr(A_.stage.binfo);
@@ -1047,7 +1047,7 @@ class Engine final : public EngineBase {
               const WarmArgs& w, std::vector<StepResult>& out, uint64_t* samples) override {
         HIP_CHECK(hipSetDevice(device_));
         const uint32_t S = (uint32_t)scenes.size();
-        const Launch L{stream_, block_for(o.block_size)};
+        const Launch L{stream_, block_for(o.block_size), (o.work_order & 128) != 0, &dev_host_->sort_abort};
         const MlpLaunch ML{stream_};
         const uint32_t digit = o.sort_digit_bits ? o.sort_digit_bits : 8u;
         const uint32_t key_bits = key_bits_for(A_.B);
@@ -1313,6 +1313,10 @@ class Engine final : public EngineBase {
             if (enq < maxit && enq <= k + lookahead) enqueue();
         }
         HIP_CHECK(hipGetLastError());
+        if (host_->sort_abort) {  // the sorts ran before the loop the host has just seen end
+            host_->sort_abort = 0;
+            throw Error(PCISPH_ERR_DEVICE, "one-sweep radix sort: a look-back wait expired; the step is invalid");
+        }
         uint64_t collected = 0;
         if (w.collect_stride) {
             const uint32_t phase = w.collect_phase % w.collect_stride;
@@ -1883,7 +1887,7 @@ int pcisph_set_solver_options(pcisph_solver* s, const pcisph_solver_options* opt
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "sort_digit_bits must be in 4..8");
         if (!(opt->neighbor_skin >= 0.f && opt->neighbor_skin <= 2.f))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "neighbor_skin must be in [0, 2]");
-        if (opt->work_order > 127) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..127");
+        if (opt->work_order > 255) throw Error(PCISPH_ERR_INVALID_ARGUMENT, "work_order must be in 0..255");
         if (opt->stage_cap && (opt->stage_cap < 64 || opt->stage_cap > kStageCap))
             throw Error(PCISPH_ERR_INVALID_ARGUMENT, "stage_cap must be 0 or in 64..2048");
         s->opt = *opt;

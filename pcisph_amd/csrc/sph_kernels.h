@@ -132,6 +132,7 @@ struct DualArrays {
 struct HostMirror {
     unsigned int running[kMaxIterationsCap];
     unsigned int tag[kMaxIterationsCap];
+    unsigned int sort_abort;  // raised by a one-sweep sort whose look-back wait expired
 };
 
 // Per-scene results, host-visible, rewritten at every iteration the scene runs, so

@@ -60,6 +60,8 @@ struct DeviceArrays {
 struct Launch {
     hipStream_t stream;
     uint32_t block;
+    bool onesweep = false;               // radix sorts: the one-sweep variant (radix_onesweep.hip)
+    unsigned int* sort_abort = nullptr;  // host-visible flag its bounded look-back raises on expiry
 };
 
 // Each launch covers the whole batch: fluid-only kernels run the fluid ranges of the
@@ -177,5 +179,10 @@ int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n
                uint32_t digit_bits, uint32_t* scratch);
 // The buffer index radix_sort returns for these sizes (known before it is enqueued).
 int radix_sort_result(uint32_t n, uint32_t key_bits, uint32_t digit_bits);
+// The one-sweep variant radix_sort runs for L.onesweep (same contract and result
+// index); its scratch need, included in radix_sort_scratch_words.
+size_t radix_onesweep_scratch_words(uint32_t n, uint32_t digit_bits);
+int radix_sort_onesweep(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t key_bits,
+                        uint32_t digit_bits, uint32_t* scratch);
 
 }  // namespace pcisph

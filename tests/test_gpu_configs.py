@@ -12,7 +12,8 @@
     size-independent properties (sort products, wall positions, finiteness);
   * the boundary-boundary density pass (pcisph.cl:195-210, never launched by the
     reference, Fluid.cpp:166-171) switched on;
-  * radix digit widths 4..8 (the sort's products are a stable sort whatever the width).
+  * radix digit widths 4..8 (the sort's products are a stable sort whatever the width),
+    for the default reduce-then-scan sort and the one-sweep variant (work_order bit 7).
 """
 import os
 
@@ -290,6 +291,48 @@ def test_1m_sort_digit_widths(gpu, digit):
     check_sort_products(g.download(Field.BOUNDARY_KEYS), g.download(Field.BOUNDARY_SRC_LOCATIONS),
                         g.download(Field.BOUNDARY_CELL_OFFSETS), scene.boundary, p.cell_size, p.bucket_count)
     g.close()
+
+
+ONESWEEP = 3 | 128  # default work order, the step's radix sorts as one-sweep
+
+
+@pytest.mark.parametrize("digit", [4, 5, 6, 7, 8])
+def test_onesweep_sort_bit_exact(gpu, oracle_lib, digit):
+    """The one-sweep sort (decoupled look-back over tile prefixes) gives the oracle's
+    stable sort, hence every field, at every digit width."""
+    scene = small_scenes()[1]
+    print(f"case one-sweep sort_digit_bits {digit}", flush=True)
+    o = oracle_lib.OracleFluid()
+    g = gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    so = run(o, scene, 2, max_iterations=20)
+    sg = run(g, scene, 2, max_iterations=20, sort_digit_bits=digit, work_order=ONESWEEP)
+    assert_same_stats(sg, so, f"one-sweep digit {digit}")
+    assert_same_state(g, o, tag=f"one-sweep digit {digit}")
+
+
+@pytest.mark.parametrize("digit", [6, 8])
+def test_1m_onesweep_sort_products(gpu, digit):
+    """At 1M (512 tiles, all resident, each waiting on the look-back of the ones
+    before it) the sort products stay the stable sort of the hash keys, and the step
+    equals the reduce-then-scan step bit for bit."""
+    scene = scenes.dam_break_1m()
+    print(f"case 1m one-sweep sort_digit_bits {digit}", flush=True)
+    runs = []
+    for wo in (3, ONESWEEP):
+        g = gpu.Fluid()
+        g.load_scene(scene, min_iterations=3, max_iterations=3, sort_digit_bits=digit, work_order=wo)
+        runs.append((g, [g.step() for _ in range(2)]))
+    g = runs[1][0]
+    p = g.get_params()
+    keys, src = g.download(Field.FLUID_KEYS), g.download(Field.FLUID_SRC_LOCATIONS)
+    assert np.array_equal(np.sort(src), np.arange(src.size, dtype=np.uint32))
+    bkeys, bsrc = g.download(Field.BOUNDARY_KEYS), g.download(Field.BOUNDARY_SRC_LOCATIONS)
+    check_sort_products(bkeys, bsrc, g.download(Field.BOUNDARY_CELL_OFFSETS), scene.boundary, p.cell_size,
+                        p.bucket_count)
+    assert_same_stats(runs[1][1], runs[0][1], "1m one-sweep")
+    assert_same_state(runs[1][0], runs[0][0], tag="1m one-sweep")
+    for r in runs:
+        r[0].close()
 
 
 # ---------------------------------------------------------------- autotuner
