@@ -749,9 +749,22 @@ __device__ __forceinline__ bool far_prunable(float d, float rho, R cell) {
     return d + rho <= kFarReach * (float)cell;
 }
 
+// One list chunk; NT: a non-temporal load, so the chunks streaming through a pass
+// do not push out of L2 the lines its gathers and the next pass reuse.
+template <bool NT>
+__device__ __forceinline__ uint3 list_chunk(const uint3* p) {
+    if constexpr (NT) {
+        typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+        const u32x3 v = __builtin_nontemporal_load(reinterpret_cast<const u32x3*>(p));
+        return make_uint3(v.x, v.y, v.z);
+    } else {
+        return *p;
+    }
+}
+
 // Particle i's list of `len` entries (chunks of four, slot-major), U loads in flight
-// per lane (U a multiple of 4).
-template <int U = kListBatch, typename Load, typename Use>
+// per lane (U a multiple of 4); NT: chunks read with non-temporal loads.
+template <int U = kListBatch, bool NT = false, typename Load, typename Use>
 __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride,
                                           uint32_t i, Load&& load, Use&& use) {
     static_assert(U % 4 == 0, "list batches are whole chunks");
@@ -764,7 +777,7 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
 #pragma unroll
         for (int c = 0; c < U / 4; ++c) {
             uint32_t e[4];
-            unpack_chunk(q[(size_t)(k / 4 + c) * stride], e);
+            unpack_chunk(list_chunk<NT>(q + (size_t)(k / 4 + c) * stride), e);
             jj[4 * c] = e[0];
             jj[4 * c + 1] = e[1];
             jj[4 * c + 2] = e[2];
@@ -777,7 +790,7 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
     }
     for (; k < len; k += 4) {
         uint32_t jj[4];
-        unpack_chunk(q[(size_t)(k / 4) * stride], jj);
+        unpack_chunk(list_chunk<NT>(q + (size_t)(k / 4) * stride), jj);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (k + u < len) use(load(jj[u]), jj[u]);
@@ -785,10 +798,10 @@ __device__ __forceinline__ void walk_list(const uint32_t* __restrict__ list, uin
 }
 
 // Particle i's list if it has one, else its 27 buckets (same order, same non-zero terms).
-template <typename R, int U = kListBatch, typename Load, typename Use>
+template <typename R, int U = kListBatch, bool NT = false, typename Load, typename Use>
 __device__ __forceinline__ void walk(const uint32_t* __restrict__ list, uint32_t len, uint32_t stride, uint32_t i,
                                      const CellTable& T, int3 c, Load&& load, Use&& use) {
-    if (len != kListOverflow) walk_list<U>(list, len, stride, i, load, use);
+    if (len != kListOverflow) walk_list<U, NT>(list, len, stride, i, load, use);
     else walk_buckets<R>(T, c, load, use);
 }
 
@@ -1448,15 +1461,18 @@ __device__ __forceinline__ void pressure_force_range(
                 fz += O::div(dz * dr * dr, r) * factor;
             }
         };
-        // positions here are x, where the lists are exact
+        // positions here are x, where the lists are exact; their chunks stream past L2
+        // (non-temporal), which keeps this pass's writes of x^ there for the next
+        // pressure pass (measured: 1M transient pressure pass -4%)
         const bool lists = NL.on != 0;
         if (P.nb) {
             const uint32_t len = lists ? NL.bcount[t] : kListOverflow;
-            walk<R>(NL.b, len, B.n_total, t, cell_table(P, bcells, B.bgrid), c,
+            walk<R, kListBatch, true>(NL.b, len, B.n_total, t, cell_table(P, bcells, B.bgrid), c,
                     [&](uint32_t j) { return bxf[j]; }, [&](const V4<R>& o, uint32_t) { add(o); });
         }
         const uint32_t len = lists ? NL.fcount[t] : kListOverflow;
-        walk<R>(NL.f, len, B.n_total, t, cell_table(P, fcells, B.fgrid), c, [&](uint32_t j) { return xf[j]; },
+        walk<R, kListBatch, true>(NL.f, len, B.n_total, t, cell_table(P, fcells, B.fgrid), c,
+                                  [&](uint32_t j) { return xf[j]; },
                 [&](const V4<R>& o, uint32_t) { add(o); });
         V4<R> f;
         f.x = fx * P.pf_scale;
