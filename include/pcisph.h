@@ -218,7 +218,8 @@ int pcisph_check_buffers_consistent(const pcisph_solver* s);
  * OpenCL device's float):
  *   0 PCISPH_PRECISION_FAST  FP32 with fused multiply-adds and hardware reciprocal /
  *                            square root (default; the accuracy class OpenCL allows
- *                            for `/`, distance() and fast_length());
+ *                            for `/`, distance() and fast_length()); a step whose
+ *                            scenes include a kernel radius above 1 m runs as IEEE;
  *   1 PCISPH_PRECISION_IEEE  FP32, one rounding per operation, the reference's order
  *                            of operations (matches the CPU restatement bit for bit);
  *   2 PCISPH_PRECISION_FP64  the same operations in double precision.

@@ -1062,10 +1062,13 @@ class Engine final : public EngineBase {
             rebuild_boxes(L, S);
             prepare_scenes(scenes, o);
         }
-        // Morton layout: bucket codes (order_code) with `obits` bits per axis and the
-        // outside-the-box bit below the scene index
+        // Morton layout: bucket codes (order_code) with `obits` bits per axis inside the
+        // scene's cell box, `out_bits` (up to 10) outside it, and the outside-the-box bit
+        // below the scene index
         const uint32_t sbits = key_bits_for(S);
         const uint32_t obits = std::min(7u, sbits < 31 ? (31u - sbits) / 3u : 0u);
+        const uint32_t out_bits = std::max(obits, std::min(10u, sbits < 30 ? (30u - sbits) / 3u : 0u));
+        const uint32_t order_bits = obits | (out_bits << 8);
         // the layout pays for its second sort only where the data outgrows the caches:
         // measured 15% slower at 64k fluid particles, 3% faster at 256k, 20% at 1M
         const bool layout =
@@ -1085,7 +1088,7 @@ class Engine final : public EngineBase {
             all_buckets = all_buckets && (!A_.NL.on || !(scene_host_[s].disp_limit > 0.f) || !n_[s]);
         const bool staged = std::is_same<R, float>::value && A_.n > 0 && !(o.work_order & 32) &&
                             ((o.work_order & 16) || all_buckets);
-        const uint32_t stage_bits = 3 * obits + 1 + sbits;
+        const uint32_t stage_bits = 3 * out_bits + 1 + sbits;
         A_.staged_pressure = staged;
         if (staged) {
             ensure_stage_buffers(fluid_blocks_);
@@ -1113,7 +1116,7 @@ class Engine final : public EngineBase {
         const bool persistent = (o.work_order & 64) && !profiled && !warm && !A_.tile_pressure;
         if (chunk) {
             std::vector<uint64_t> key =
-                launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, obits, w.params, staged,
+                launch_key(L, fast, warm, features, digit, key_bits, maxit, chunk, layout_bits, order_bits, w.params, staged,
                            w.collect_stride != 0);
             if (key != graph_key_) {
                 drop_graphs();
@@ -1147,7 +1150,7 @@ class Engine final : public EngineBase {
         if (staged && A_.nb) {
             const std::vector<uint64_t> bk{digit, stage_bits, A_.nb, (uint64_t)(uintptr_t)A_.stage.bwork};
             if (bk != bwork_key_) {
-                const int r = DataLaunch<R>::boundary_order(L, A_, obits, stage_bits, digit, sort_scratch_,
+                const int r = DataLaunch<R>::boundary_order(L, A_, order_bits, stage_bits, digit, sort_scratch_,
                                                             bstage_keys_, bstage_vals_);
                 if (r < 0 || bstage_vals_[r] != A_.stage.bwork)
                     throw Error(PCISPH_ERR_INTERNAL, "boundary work order not where expected");
@@ -1178,7 +1181,7 @@ class Engine final : public EngineBase {
                         a.okeys[k] = okeys_[k];
                         a.ovals[k] = ovals_[k];
                     }
-                    a.bits = obits;
+                    a.bits = obits | (obits << 8);
                     a.key_bits = layout_bits;
                     a.digit_bits = digit;
                     a.scratch = sort_scratch_;
@@ -1195,7 +1198,7 @@ class Engine final : public EngineBase {
                 }
                 if (A_.box_cells) DataLaunch<R>::fill_box(L, scene_table_, S, box_max_, A_.cells, A_.fgrid);
                 if (staged)
-                    DataLaunch<R>::stage_setup(L, A_, obits, stage_bits, digit, sort_scratch_, stage_keys_,
+                    DataLaunch<R>::stage_setup(L, A_, order_bits, stage_bits, digit, sort_scratch_, stage_keys_,
                                                stage_vals_);
             });
             prof_.timed(PCISPH_STAGE_DENSITY, stream_, -1, [&] { ops.density(L, A_); });
@@ -2075,9 +2078,13 @@ static void step_impl(pcisph_solver* s, std::vector<pcisph_step_stats>* out) {
         in.push_back({&sc.params, sc.threshold, w.params != nullptr && !sc.warm_skip});
         sort_boundary = sort_boundary || sc.boundary_updated;
     }
+    // the fast kernels' clamped poly6 term needs h^2 <= 1 (kernel radius at most 1 m);
+    // a batch with a larger one runs the IEEE FP32 kernels
+    bool fast = s->precision == kPrecisionFast;
+    for (const auto& sc : s->scenes) fast = fast && sc.params.kernel_radius2 <= 1.f;
     std::vector<StepResult> res;
     uint64_t samples = 0;
-    s->eng->step(in, s->opt, sort_boundary, s->precision == kPrecisionFast, w, res, &samples);
+    s->eng->step(in, s->opt, sort_boundary, fast, w, res, &samples);
     s->samples_count = std::min(s->samples_cap, s->samples_count + samples);
     for (auto& sc : s->scenes) sc.boundary_updated = false;
     for (size_t i = 0; i < res.size() && i < s->scenes.size(); ++i)

@@ -107,8 +107,8 @@ __global__ void k_translate(V4<R>* __restrict__ p, uint32_t n, R dx, R dy, R dz)
 // Part 1, over canonical indices c of the sorted keys K (S[c] = the particle's slot
 // before this step): the reference's source location of c (the canonical index the
 // particle had before the sort) and, at the first c of each bucket, the bucket's
-// layout code: scene index above the Morton code of that particle's cell (cell
-// coordinates taken modulo 2^bits).
+// layout code: the order code of that particle's cell (order_code, `bits` its packed
+// widths).
 template <typename R>
 __global__ void k_layout_codes(const SphParams<R>* __restrict__ scenes, const BlockRange* __restrict__ blocks,
                                const uint32_t* __restrict__ K, const uint32_t* __restrict__ S,

@@ -175,23 +175,31 @@ struct Batch {
     const uint2* bgrid;
 };
 
-// Code of a cell for the storage and work orders: the Morton interleave of `bits`
-// bits per axis of its coordinates -- relative to the scene's cell box when it has
-// one -- with the scene index above.  Relative to the box, cells outside its first
-// 2^bits cells along some axis (particles thrown clear of the scene) get the bit above
-// the Morton code, so they follow every cell inside instead of wrapping round onto
-// it; without a box the coordinates wrap modulo 2^bits.  3 bits + 1 + scene bits.
+// Code of a cell for the storage and work orders.  `bits` packs two widths: the low
+// byte ib, the bits per axis of a cell inside the scene's cell box, and the next byte
+// ob >= ib, the bits per axis of a cell outside it.  Inside the box (its first 2^ib
+// cells along each axis) the code is the Morton interleave of the box-relative
+// coordinates; a cell outside (particles thrown clear of the scene) gets the bit above
+// an ob-bit Morton code of its box-relative coordinates offset by 2^(ob-1), so it
+// follows every cell inside, and clumps of such particles within 2^(ob-1) cells of the
+// box stay together instead of wrapping onto one another every 2^ib cells.  Without a
+// box the coordinates wrap modulo 2^ib.  The scene index sits above: 3 ob + 1 + scene
+// bits.
 template <typename R>
 __device__ __forceinline__ uint32_t order_code(const SphParams<R>& P, uint32_t scene, int3 q, uint32_t bits) {
-    uint32_t x = (uint32_t)q.x, y = (uint32_t)q.y, z = (uint32_t)q.z, out = 0;
+    const uint32_t ib = bits & 0xFFu, ob = bits >> 8;
+    uint32_t x = (uint32_t)q.x, y = (uint32_t)q.y, z = (uint32_t)q.z;
     if (P.box_dim[0]) {
         x -= P.box_lo[0];
         y -= P.box_lo[1];
         z -= P.box_lo[2];
-        out = ((x | y | z) >> bits) ? 1u : 0u;
+        if ((x | y | z) >> ib) {
+            const uint32_t half = ob ? 1u << (ob - 1) : 0u;
+            return (scene << (3 * ob + 1)) | (1u << (3 * ob)) | spread_bits3(x + half, ob) |
+                   (spread_bits3(y + half, ob) << 1) | (spread_bits3(z + half, ob) << 2);
+        }
     }
-    return (scene << (3 * bits + 1)) | (out << (3 * bits)) | spread_bits3(x, bits) | (spread_bits3(y, bits) << 1) |
-           (spread_bits3(z, bits) << 2);
+    return (scene << (3 * ob + 1)) | spread_bits3(x, ib) | (spread_bits3(y, ib) << 1) | (spread_bits3(z, ib) << 2);
 }
 
 // A scene's fluid (or boundary) bucket ranges: the batch's bucket table and grid.
@@ -322,10 +330,13 @@ __device__ __forceinline__ R poly6(R r2, R h2) {
 // pcisph.cl:238-252 over poly6, pcisph.cl:8-16).  IEEE FP32 and FP64: the reference's
 // operations -- r^2, the branch at h^2, (h^2 - r^2)^3 -- one rounding each.  Fast FP32:
 // h^2 - r^2 by three fused multiply-adds from h^2, clamped at zero and cubed into the
-// sum by one more (nine vector operations per candidate instead of twelve); the term
-// is the same to within fused rounding and exactly zero beyond h, as the branch makes
-// it.  A NaN distance adds zero there; the pass's max(0, rho* - rho0) drops a NaN sum
-// in every mode.
+// sum by one more; the term is the same to within fused rounding and exactly zero
+// beyond h, as the branch makes it.  The clamp is taken to [0, 1], which the compiler
+// folds into the last multiply-add's output modifier (eight vector operations per
+// candidate instead of twelve): it is max(0, h^2 - r^2) because h^2 - r^2 <= h^2 <= 1,
+// and the solver runs the fast kernels only where every scene has h <= 1 m (FP32 IEEE
+// otherwise).  A NaN distance adds zero there; the pass's max(0, rho* - rho0) drops a
+// NaN sum in every mode.
 template <typename R, bool FAST>
 __device__ __forceinline__ R poly6_add(R sum, const V4<R>& a, const V4<R>& b, R h2) {
     if constexpr (FAST && sizeof(R) == 4) {
@@ -333,7 +344,7 @@ __device__ __forceinline__ R poly6_add(R sum, const V4<R>& a, const V4<R>& b, R 
         float e = __builtin_fmaf(-dx, dx, h2);
         e = __builtin_fmaf(-dy, dy, e);
         e = __builtin_fmaf(-dz, dz, e);
-        e = __builtin_fmaxf(e, 0.f);
+        e = __builtin_fminf(__builtin_fmaxf(e, 0.f), 1.f);
         return __builtin_fmaf(e * e, e, sum);
     } else {
         R dx, dy, dz;

@@ -98,7 +98,7 @@ struct LayoutArgs {
     uint32_t* code;   // bucket codes, one word per bucket of the batch
     uint32_t* okeys[2];
     uint32_t* ovals[2];
-    uint32_t bits, key_bits, digit_bits;
+    uint32_t bits, key_bits, digit_bits;  // bits: order_code's packed widths (inside | outside << 8)
     uint32_t* scratch;
     const V4<R>* pin;
     const V4<R>* vin;
@@ -148,8 +148,8 @@ struct DataLaunch {
     // (k_extents; ext prepared by the caller)
     static void extents(const Launch& L, const Batch<R>& B, const V4<R>* pos, const V4<R>* bpos, int* ext);
     // the per-step staging of the pressure pass (sph_staged.h): the work order -- the
-    // fluid slots sorted stably by the Morton code of their own cell (`bits` per axis,
-    // scene index above) through keys/vals -- then each fluid range's staging map and
+    // fluid slots sorted stably by the order code of their own cell (order_code, `bits`
+    // its packed widths) through keys/vals -- then each fluid range's staging map and
     // row descriptors into A.stage (whose `work` must be vals[radix_sort_result(...)])
     static void stage_setup(const Launch& L, const DeviceArrays<R>& A, uint32_t bits, uint32_t key_bits,
                             uint32_t digit_bits, uint32_t* scratch, uint32_t* keys[2], uint32_t* vals[2]);

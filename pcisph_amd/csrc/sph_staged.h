@@ -301,22 +301,24 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
 }
 
 // The poly6 terms of one staged row, LDS candidates [a, a + n) in order (x and y in
-// `xy`, z in `z`), added to `pd`.  Four candidates per step from two LDS pointers:
-// paired loads and no per-candidate address arithmetic.
+// `xy`, z in `z`), added to `pd`.  Four candidates per step from two LDS pointers
+// (paired loads, constant offsets), each step one add per pointer and one compare
+// against the row's last full step.
 template <bool FAST>
 __device__ __forceinline__ float staged_row(const float2* __restrict__ xy, const float* __restrict__ z, uint32_t a,
                                             uint32_t n, const float4& pr, float h2, float pd) {
     const float2* q = xy + a;
     const float* qz = z + a;
+    const float2* const end4 = q + (n & ~3u);
     const float2* const end = q + n;
-    for (; q + 4 <= end; q += 4, qz += 4) {
+    for (; q != end4; q += 4, qz += 4) {
         float4 o[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) o[u] = make_float4(q[u].x, q[u].y, qz[u], 0.f);
 #pragma unroll
         for (int u = 0; u < 4; ++u) pd = poly6_add<float, FAST>(pd, pr, o[u], h2);
     }
-    for (; q < end; ++q, ++qz) pd = poly6_add<float, FAST>(pd, pr, make_float4(q->x, q->y, *qz, 0.f), h2);
+    for (; q != end; ++q, ++qz) pd = poly6_add<float, FAST>(pd, pr, make_float4(q->x, q->y, *qz, 0.f), h2);
     return pd;
 }
 
