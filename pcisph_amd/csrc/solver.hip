@@ -1062,13 +1062,14 @@ class Engine final : public EngineBase {
             rebuild_boxes(L, S);
             prepare_scenes(scenes, o);
         }
-        // Morton layout: bucket codes (order_code) with `obits` bits per axis inside the
-        // scene's cell box, `out_bits` (up to 10) outside it, and the outside-the-box bit
-        // below the scene index
+        // Order codes (order_code): `obits` bits per axis, the outside-the-box bit below
+        // the scene index.  The Morton layout's bucket codes take cells outside the box
+        // modulo 2^obits; the staged pressure pass's work order takes them in blocks of
+        // 8 cells along each axis (2^(obits+2) cells either side of the box), which keeps
+        // clumps thrown clear of the scene together at the same key width
         const uint32_t sbits = key_bits_for(S);
         const uint32_t obits = std::min(7u, sbits < 31 ? (31u - sbits) / 3u : 0u);
-        const uint32_t out_bits = std::max(obits, std::min(10u, sbits < 30 ? (30u - sbits) / 3u : 0u));
-        const uint32_t order_bits = obits | (out_bits << 8);
+        const uint32_t order_bits = obits | (obits << 8) | (3u << 16);
         // the layout pays for its second sort only where the data outgrows the caches:
         // measured 15% slower at 64k fluid particles, 3% faster at 256k, 20% at 1M
         const bool layout =
@@ -1088,7 +1089,7 @@ class Engine final : public EngineBase {
             all_buckets = all_buckets && (!A_.NL.on || !(scene_host_[s].disp_limit > 0.f) || !n_[s]);
         const bool staged = std::is_same<R, float>::value && A_.n > 0 && !(o.work_order & 32) &&
                             ((o.work_order & 16) || all_buckets);
-        const uint32_t stage_bits = 3 * out_bits + 1 + sbits;
+        const uint32_t stage_bits = 3 * obits + 1 + sbits;
         A_.staged_pressure = staged;
         if (staged) {
             ensure_stage_buffers(fluid_blocks_);
@@ -1181,7 +1182,7 @@ class Engine final : public EngineBase {
                         a.okeys[k] = okeys_[k];
                         a.ovals[k] = ovals_[k];
                     }
-                    a.bits = obits | (obits << 8);
+                    a.bits = obits | (obits << 8);  // outside the box: modulo 2^obits
                     a.key_bits = layout_bits;
                     a.digit_bits = digit;
                     a.scratch = sort_scratch_;

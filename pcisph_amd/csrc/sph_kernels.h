@@ -175,19 +175,21 @@ struct Batch {
     const uint2* bgrid;
 };
 
-// Code of a cell for the storage and work orders.  `bits` packs two widths: the low
-// byte ib, the bits per axis of a cell inside the scene's cell box, and the next byte
-// ob >= ib, the bits per axis of a cell outside it.  Inside the box (its first 2^ib
-// cells along each axis) the code is the Morton interleave of the box-relative
-// coordinates; a cell outside (particles thrown clear of the scene) gets the bit above
-// an ob-bit Morton code of its box-relative coordinates offset by 2^(ob-1), so it
-// follows every cell inside, and clumps of such particles within 2^(ob-1) cells of the
-// box stay together instead of wrapping onto one another every 2^ib cells.  Without a
-// box the coordinates wrap modulo 2^ib.  The scene index sits above: 3 ob + 1 + scene
-// bits.
+// Code of a cell for the storage and work orders.  `bits` packs three fields: the low
+// byte ib, the bits per axis of a cell inside the scene's cell box; the next byte ob,
+// the bits per axis of a cell outside it; the third byte sh, a right shift of the
+// coordinates of a cell outside.  Inside the box (its first 2^ib cells along each
+// axis) the code is the Morton interleave of the box-relative coordinates; a cell
+// outside (particles thrown clear of the scene) gets the bit above an ob-bit Morton
+// code of its box-relative coordinates divided by 2^sh (floor) and offset by 2^(ob-1),
+// so it follows every cell inside, and a clump of such particles stays together (in
+// blocks of 2^sh cells, within 2^(ob-1+sh) cells of the box) instead of wrapping onto
+// others every 2^ib cells.  Without a box the coordinates wrap modulo 2^ib.  The scene
+// index sits above: max(ib, ob) * 3 + 1 + scene bits.
 template <typename R>
 __device__ __forceinline__ uint32_t order_code(const SphParams<R>& P, uint32_t scene, int3 q, uint32_t bits) {
-    const uint32_t ib = bits & 0xFFu, ob = bits >> 8;
+    const uint32_t ib = bits & 0xFFu, ob = (bits >> 8) & 0xFFu, sh = bits >> 16;
+    const uint32_t w = ib > ob ? ib : ob;
     uint32_t x = (uint32_t)q.x, y = (uint32_t)q.y, z = (uint32_t)q.z;
     if (P.box_dim[0]) {
         x -= P.box_lo[0];
@@ -195,11 +197,13 @@ __device__ __forceinline__ uint32_t order_code(const SphParams<R>& P, uint32_t s
         z -= P.box_lo[2];
         if ((x | y | z) >> ib) {
             const uint32_t half = ob ? 1u << (ob - 1) : 0u;
-            return (scene << (3 * ob + 1)) | (1u << (3 * ob)) | spread_bits3(x + half, ob) |
-                   (spread_bits3(y + half, ob) << 1) | (spread_bits3(z + half, ob) << 2);
+            const uint32_t cx = (uint32_t)((int32_t)x >> sh) + half, cy = (uint32_t)((int32_t)y >> sh) + half,
+                           cz = (uint32_t)((int32_t)z >> sh) + half;
+            return (scene << (3 * w + 1)) | (1u << (3 * w)) | spread_bits3(cx, ob) | (spread_bits3(cy, ob) << 1) |
+                   (spread_bits3(cz, ob) << 2);
         }
     }
-    return (scene << (3 * ob + 1)) | spread_bits3(x, ib) | (spread_bits3(y, ib) << 1) | (spread_bits3(z, ib) << 2);
+    return (scene << (3 * w + 1)) | spread_bits3(x, ib) | (spread_bits3(y, ib) << 1) | (spread_bits3(z, ib) << 2);
 }
 
 // A scene's fluid (or boundary) bucket ranges: the batch's bucket table and grid.
