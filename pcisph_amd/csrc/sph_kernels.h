@@ -820,6 +820,31 @@ __device__ __forceinline__ void walk(const uint32_t* __restrict__ list, uint32_t
     else walk_buckets<R>(T, c, load, use);
 }
 
+// A boundary particle's share of the density pass: its pressure reset
+// (boundary_pressure_initialization, pcisph.cl:114-119) and, where pressure passes test
+// their lists, its list of fluid neighbours.
+template <typename R>
+__device__ __forceinline__ void density_boundary_particle(const Batch<R>& B, const SphParams<R>& P, uint32_t b,
+                                                          const CellTable& FT, const V4<R>* __restrict__ bpos,
+                                                          const V4<R>* __restrict__ pos, R* __restrict__ bp,
+                                                          V4<R>* __restrict__ bxf, const NeighborLists& NL) {
+    bp[b] = 0;
+    V4<R> o = bpos[b];
+    o.w = 0;
+    bxf[b] = o;
+    if (NL.on && P.disp_limit > 0.f) {  // read only by pressure passes that test their lists
+        const R lr2 = (R)P.list_r2;
+        const int3 c = cell_of4<R>(o, P.cell);
+        ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0, {}};
+        walk_buckets<R>(FT, c, [&](uint32_t j) { return pos[j]; },
+                        [&](const V4<R>& q, uint32_t j) {
+                            R dx, dy, dz;
+                            if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
+                        });
+        NL.bfcount[b] = w.finish();
+    }
+}
+
 // FP32 density passes are built for 8 wavefronts per SIMD (64 VGPRs: a 24-byte spill,
 // measured 5-10% faster than 7 wavefronts at 68 VGPRs); FP64 keeps its registers.
 template <typename R>
@@ -892,21 +917,7 @@ void k_density(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __rest
             cb_sum = cb;
         }
     } else if (in) {
-        const uint32_t b = t;
-        bp[b] = 0;
-        V4<R> o = bpos[b];
-        o.w = 0;
-        bxf[b] = o;
-        if (lists && P.disp_limit > 0.f) {  // read only by pressure passes that test their lists
-            const int3 c = cell_of4<R>(o, P.cell);
-            ListWriter w{NL.bf, B.nb_total, b, NL.cap_bf, 0, {}};
-            walk_buckets<R>(FT, c, [&](uint32_t j) { return pos[j]; },
-                            [&](const V4<R>& q, uint32_t j) {
-                                R dx, dy, dz;
-                                if (dist2<R>(o.x, o.y, o.z, q.x, q.y, q.z, dx, dy, dz) <= lr2) w.push(j);
-                            });
-            NL.bfcount[b] = w.finish();
-        }
+        density_boundary_particle<R>(B, P, t, FT, bpos, pos, bp, bxf, NL);
     }
     const unsigned long long a = block_reduce_sum(cf_sum, lds);
     __syncthreads();

@@ -15,6 +15,14 @@ void SphLaunch<R, FAST>::density(const Launch& L, const DeviceArrays<R>& A) {
     const Batch<R>& B = A.batch;
     const uint32_t grid = B.fluid_blocks + B.boundary_blocks;
     if (!grid) return;
+    if constexpr (std::is_same<R, float>::value) {
+        if (A.staged_pressure) {
+            // fluid ranges from the step's LDS staging (set up before this pass)
+            hipLaunchKernelGGL((k_density_staged<FAST>), dim3(grid), dim3(L.block), 0, L.stream, B, A.bcells,
+                               A.bpos, A.cells, A.pos, A.xr, A.counts, A.bp, A.bxf, A.NL, A.stage);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_density<R, FAST>), dim3(grid), dim3(L.block), 0, L.stream, B, A.bcells, A.bpos, A.cells,
                        A.pos, A.xr, A.counts, A.bp, A.bxf, A.NL);
 }

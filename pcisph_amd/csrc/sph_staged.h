@@ -336,6 +336,171 @@ __device__ __forceinline__ float global_row(const CellTable& T, int3 c, int row,
     return pd;
 }
 
+// The candidates of one staged row as a walk visits them: LDS entries [a, a + n) of
+// `s` (x, y, z and the candidate's slot in w), use(o, slot) in order, four at a time.
+template <typename Use>
+__device__ __forceinline__ void staged_row_each(const float4* __restrict__ s, uint32_t a, uint32_t n, Use&& use) {
+    const float4* q = s + a;
+    const float4* const end4 = q + (n & ~3u);
+    const float4* const end = q + n;
+    for (; q != end4; q += 4) {
+        float4 o[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) o[u] = q[u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) use(o[u], __float_as_uint(o[u].w));
+    }
+    for (; q != end; ++q) {
+        const float4 o = *q;
+        use(o, __float_as_uint(o.w));
+    }
+}
+
+// Row `row` of cell c's 27 (dz-major, then dy) walked in global memory: its three
+// bucket ranges from the cell table, candidates through load(j), use(data, j) in order.
+template <typename Load, typename Use>
+__device__ __forceinline__ void global_row_each(const CellTable& T, int3 c, int row, Load&& load, Use&& use) {
+    const uint32_t x = (uint32_t)c.x - 1u, y = (uint32_t)c.y + (uint32_t)(row % 3 - 1),
+                   z = (uint32_t)c.z + (uint32_t)(row / 3 - 1);
+    gather_row<kGatherBatch>(cell_range(T, x, y, z), cell_range(T, x + 1u, y, z), cell_range(T, x + 2u, y, z), load,
+                             use);
+}
+
+// update_density (pcisph.cl:65-92) of the FP32 step, with the same counts and neighbour
+// lists as k_density, its fluid ranges walking the step's LDS staging (see above) at
+// the positions x instead of the 27 buckets in global memory: the workgroup copies its
+// staged candidates' positions and slots into LDS (x, y, z, slot), then each particle
+// of its work order walks its nine rows there -- the reference's candidates in the
+// reference's order, so the sums, counts and lists are the global walk's bit for bit.
+// Its boundary walk reads the staged boundary rows where a pruning walk may (the
+// candidate count still takes the pruned buckets, as walk_buckets' `full`), global
+// memory otherwise; a partly staged workgroup walks the rows past the staging in
+// global memory, a workgroup without staging all of them.  Boundary ranges as
+// k_density.
+template <bool FAST>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_density_staged(
+    Batch<float> B, const uint2* __restrict__ bcells, const float4* __restrict__ bpos,
+    const uint2* __restrict__ fcells, const float4* __restrict__ pos, float4* __restrict__ xr,
+    uint4* __restrict__ counts, float* __restrict__ bp, float4* __restrict__ bxf, NeighborLists NL, StageMaps S) {
+    __shared__ float4 s_x[kStageCap];
+    __shared__ unsigned long long lds[16];
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<float> P = B.scenes[rg.scene];
+    LoopState* st = B.states + rg.scene;
+    const CellTable FT = cell_table(P, fcells, B.fgrid);
+    const bool lists = NL.on != 0;
+    const float lr2 = P.list_r2;
+    if (lists) clear_table(NL.disp[0], NL.buckets);  // filled by the force pass
+    unsigned long long cf_sum = 0, cb_sum = 0;
+    if (rg.flags & kRangeBoundary) {
+        const uint32_t t = rg.begin + threadIdx.x;
+        if (t < rg.end) density_boundary_particle<float>(B, P, t, FT, bpos, pos, bp, bxf, NL);
+    } else {
+        const uint32_t tile = rg.flags >> kRangeTileShift;
+        const uint32_t total = S.info[tile];
+        const bool staged = total != kStageFallback;
+        const uint32_t btotal = staged ? S.binfo[tile] : kStageFallback;
+        const bool bstaged = btotal != kStageFallback;
+        const uint32_t fitted = staged ? min(total, S.cap) : 0u;
+        if (staged) {
+            const uint32_t all = fitted + (bstaged ? btotal : 0u);
+            const uint32_t* src = S.src + (size_t)tile * kStageCap;
+            for (uint32_t q0 = threadIdx.x; q0 < all; q0 += kCopyBatch * blockDim.x) {
+                uint32_t j[kCopyBatch];
+                float4 v[kCopyBatch];
+                const float4* from[kCopyBatch];
+#pragma unroll
+                for (int u = 0; u < kCopyBatch; ++u) {
+                    const uint32_t q = q0 + u * blockDim.x, qq = q < all ? q : q0;
+                    j[u] = src[qq];
+                    from[u] = qq < fitted ? pos : bpos;
+                }
+#pragma unroll
+                for (int u = 0; u < kCopyBatch; ++u) v[u] = from[u][j[u]];
+#pragma unroll
+                for (int u = 0; u < kCopyBatch; ++u) {
+                    const uint32_t q = q0 + u * blockDim.x;
+                    if (q < all) s_x[q] = make_float4(v[u].x, v[u].y, v[u].z, __uint_as_float(j[u]));
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t k = rg.begin + threadIdx.x;
+        if (k < rg.end) {
+            const uint32_t i = S.work[k];
+            const float4 self = pos[i];
+            const int3 c = cell_of4<float>(self, P.cell);
+            float rho = 0.f;
+            uint32_t cf = 0, cb = 0, nf = 0, nb = 0;
+            ListWriter wb{NL.b, B.n_total, i, NL.cap_b, 0, {}}, wf{NL.f, B.n_total, i, NL.cap_f, 0, {}};
+            // the terms and counts of k_density, candidate by candidate
+            auto bnd = [&](const float4& o, uint32_t j) {
+                float dx, dy, dz;
+                const float r2 = dist2<float>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+                rho += poly6<float>(r2, P.h2);
+                nb += !(r2 > P.h2);
+                if (lists && r2 <= lr2) wb.push(j);
+            };
+            auto fl = [&](const float4& o, uint32_t j) {
+                float dx, dy, dz;
+                const float r2 = dist2<float>(self.x, self.y, self.z, o.x, o.y, o.z, dx, dy, dz);
+                rho += poly6<float>(r2, P.h2);
+                ++cf;
+                nf += !(r2 > P.h2);
+                if (lists && r2 <= lr2) wf.push(j);
+            };
+            auto bload = [&](uint32_t j) { return bpos[j]; };
+            auto fload = [&](uint32_t j) { return pos[j]; };
+            if (P.nb) {
+                // terms vanish beyond h; list entries beyond the list radius
+                CellTable BT = cell_table(P, bcells, B.bgrid);
+                BT.prune = far_prunable<float>(0.f, sqrtf(fmaxf((float)P.h2, lists ? lr2 : 0.f)), P.cell);
+                if (BT.prune && bstaged) {
+                    for_each_range_row(BT, c, [](uint2, uint2, uint2) {}, &cb);
+                    if (btotal) {
+#pragma unroll 1
+                        for (int row = 0; row < 9; ++row) {
+                            const uint32_t d = S.brows[(size_t)row * B.n_total + k];
+                            staged_row_each(s_x, d & 0xFFFFu, d >> 16, bnd);
+                        }
+                    }
+                } else {
+                    walk_buckets<float>(BT, c, bload, bnd, &cb);
+                }
+            }
+            if (staged) {
+#pragma unroll 1
+                for (int row = 0; row < 9; ++row) {
+                    const uint32_t d = S.rows[(size_t)row * B.n_total + k];
+                    const uint32_t a = d & 0xFFFFu, n = d >> 16;
+                    if (a + n <= fitted) staged_row_each(s_x, a, n, fl);
+                    else global_row_each(FT, c, row, fload, fl);
+                }
+            } else {
+                walk_buckets<float>(FT, c, fload, fl);
+            }
+            rho *= P.density_scale;
+            float4 out = self;
+            out.w = rho;
+            xr[i] = out;
+            if (counts) counts[i] = make_uint4(cf, cb, nf, nb);
+            if (lists) {
+                NL.bcount[i] = P.nb ? wb.finish() : 0u;
+                NL.fcount[i] = wf.finish();
+            }
+            cf_sum = cf;
+            cb_sum = cb;
+        }
+    }
+    const unsigned long long a = block_reduce_sum(cf_sum, lds);
+    __syncthreads();
+    const unsigned long long b = block_reduce_sum(cb_sum, lds + 8);
+    if (threadIdx.x == 0 && (a || b)) {
+        atomicAdd(&st->fluid_candidates, a);
+        atomicAdd(&st->boundary_candidates, b);
+    }
+}
+
 // update_pressure (pcisph.cl:212-263), the fluid ranges with their candidates from the
 // workgroup's LDS staging (see above) -- the same results as k_pressure's bucket walk --
 // and the boundary ranges (boundary mode) walking their fluid buckets in global
