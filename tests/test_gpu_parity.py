@@ -463,3 +463,28 @@ def test_layout_transitions_bit_exact(gpu, oracle_lib):
     g.restore()
     step_both(1)
     same("restore")
+
+
+def test_fast_mode_above_one_metre_runs_ieee(gpu, oracle_lib):
+    """The fast kernels clamp the poly6 term to [0, 1] in one instruction, which is
+    max(0, h^2 - r^2) only while h^2 <= 1; a scene with a kernel radius above 1 m (here
+    1.2 m: the 8^3 dam break scaled by 30) runs the IEEE FP32 kernels in fast mode, so
+    its steps equal an IEEE run's and the oracle's bit for bit."""
+    scene = scenes.dam_break((8, 8, 8), 0.6, 0.03, (12.0, 12.0, 9.0), seed=1234, name="dam_break_x30", steps=2)
+    print(f"case fast_above_1m {scene.name} N={scene.fluid_count} Nb={scene.boundary_count}", flush=True)
+    fast, ieee = gpu.Fluid(), gpu.Fluid(precision=gpu.PRECISION_IEEE)
+    o = oracle_lib.OracleFluid()
+    sf = run(fast, scene, 2, max_iterations=20)
+    si = run(ieee, scene, 2, max_iterations=20)
+    so = run(o, scene, 2, max_iterations=20)
+    assert fast.get_params().kernel_radius > 1.0
+    assert [s["iterations"] for s in sf] == [s["iterations"] for s in si] == [s["iterations"] for s in so]
+    rf, ri, ro = snapshot(fast), snapshot(ieee), snapshot(o)
+    for name in ri:
+        np.testing.assert_array_equal(rf[name], ri[name], err_msg=name)
+        np.testing.assert_array_equal(rf[name], ro[name], err_msg=name)
+    # and a scene at h <= 1 m does run the fast kernels: its fields differ from IEEE
+    small = small_scenes()[0]
+    run(fast, small, 1, max_iterations=20)
+    run(ieee, small, 1, max_iterations=20)
+    assert not np.array_equal(fast.download(Field.FLUID_DENSITIES), ieee.download(Field.FLUID_DENSITIES))
