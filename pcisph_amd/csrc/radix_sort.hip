@@ -7,8 +7,10 @@
 // Per digit pass:
 //   upsweep   : each tile of 2048 keys builds its digit histogram in LDS and writes
 //               it digit-major, hist[d * tiles + t]
-//   scan      : hist becomes exclusive global offsets (three launches: chunk scans,
-//               scan of the chunk totals, add-back)
+//   scan      : hist becomes exclusive offsets within chunks of 2048 words, the
+//               chunk totals beside them (one launch); with at most kFusedChunks
+//               chunks each downsweep workgroup scans the totals itself and adds them
+//               back (else two more launches finish the scan first)
 //   downsweep : each tile re-reads its keys, ranks them stably with wavefront
 //               ballots (the lanes sharing a digit are found with one ballot per
 //               digit bit; a lane's rank is the popcount of those below it), adds
@@ -109,21 +111,48 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* __restrict_
         if (base + k < count) data[base + k] += add;
 }
 
+// Chunk totals a downsweep workgroup scans itself (8 per thread).
+constexpr uint32_t kFusedChunks = 2048;
+
+// `offsets`: exclusive global offsets, or (with `partials`) offsets within the scan's
+// chunks, `partials` holding the `chunks` chunk totals.
 __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin,
                                                          uint32_t* __restrict__ kout,
                                                          uint32_t* __restrict__ vout, uint32_t n,
                                                          uint32_t shift, uint32_t bits,
                                                          const uint32_t* __restrict__ offsets,
-                                                         uint32_t tiles) {
+                                                         uint32_t tiles, const uint32_t* __restrict__ partials,
+                                                         uint32_t chunks) {
     __shared__ uint32_t s_run[kMaxRadix];            // next free global slot per digit
     __shared__ uint32_t s_wave[kWaves][kMaxRadix];   // per-wavefront count, then base
+    __shared__ uint32_t s_pre[kFusedChunks];         // exclusive prefix of the chunk totals
+    __shared__ uint32_t s_tmp[kWaves];
     const uint32_t radix = 1u << bits, mask = radix - 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
-        s_run[d] = offsets[d * tiles + blockIdx.x];
+    if (partials) {
+        constexpr uint32_t per = kFusedChunks / kThreads;
+        const uint32_t c0 = threadIdx.x * per;
+        uint32_t v[per], sum = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) s_wave[w][d] = 0;
+        for (uint32_t k = 0; k < per; ++k) {
+            v[k] = c0 + k < chunks ? partials[c0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = block_exclusive_scan(sum, s_tmp, total);
+#pragma unroll
+        for (uint32_t k = 0; k < per; ++k) {
+            s_pre[c0 + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+    }
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
+        const uint32_t w = d * tiles + blockIdx.x;
+        s_run[d] = offsets[w] + (partials ? s_pre[w / kScanChunk] : 0u);
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) s_wave[k][d] = 0;
     }
     __syncthreads();
     const unsigned long long lt = (1ull << lane) - 1ull;
@@ -201,13 +230,20 @@ size_t scan_partial_words(size_t count) {
     return words + 1024;
 }
 
-void exclusive_scan(const Launch& L, uint32_t* data, uint32_t count, uint32_t* partials) {
+// Exclusive scan of `count` words.  With `leave_chunks`, the scan stops after the
+// chunk scans when there are at most kFusedChunks chunks and returns their count
+// (`data` then holds offsets within chunks, `partials` the chunk totals); otherwise
+// it completes and returns 0.
+uint32_t exclusive_scan(const Launch& L, uint32_t* data, uint32_t count, uint32_t* partials,
+                        bool leave_chunks = false) {
     const uint32_t chunks = (count + kScanChunk - 1) / kScanChunk;
     hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(kScanThreads), 0, L.stream, data, count, partials);
-    if (chunks <= 1) return;
+    if (leave_chunks && chunks <= kFusedChunks) return chunks;
+    if (chunks <= 1) return 0;
     if (chunks <= 1024) hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, L.stream, partials, chunks);
     else exclusive_scan(L, partials, chunks, partials + chunks);
     hipLaunchKernelGGL(k_scan_add, dim3(chunks), dim3(kScanThreads), 0, L.stream, data, count, partials);
+    return 0;
 }
 
 size_t radix_sort_scratch_words(uint32_t n, uint32_t digit_bits) {
@@ -234,9 +270,11 @@ int radix_sort(const Launch& L, uint32_t* keys[2], uint32_t* vals[2], uint32_t n
         const uint32_t words = (1u << bits) * tiles;
         hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], n, shift, bits,
                            scratch, tiles);
-        exclusive_scan(L, scratch, words, scratch + (size_t(1) << digit_bits) * tiles);
+        uint32_t* partials = scratch + (size_t(1) << digit_bits) * tiles;
+        const uint32_t chunks = exclusive_scan(L, scratch, words, partials, true);
         hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kThreads), 0, L.stream, keys[cur], vals[cur],
-                           keys[cur ^ 1], vals[cur ^ 1], n, shift, bits, scratch, tiles);
+                           keys[cur ^ 1], vals[cur ^ 1], n, shift, bits, scratch, tiles,
+                           chunks ? partials : nullptr, chunks);
         cur ^= 1;
     }
     return cur;
