@@ -115,7 +115,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(uint32_t* __restrict_
 constexpr uint32_t kFusedChunks = 2048;
 
 // `offsets`: exclusive global offsets, or (with `partials`) offsets within the scan's
-// chunks, `partials` holding the `chunks` chunk totals.
+// chunks, `partials` holding the `chunks` chunk totals.  The tile is ranked whole
+// before anything is written: every item's rank among the lanes of its wavefront that
+// hold the same digit (one ballot per digit bit), digit counts per (round, wavefront),
+// their scan in element order per digit and over the digits, then the tile in its
+// stable sorted order in LDS, written out in runs of one digit (consecutive threads,
+// consecutive global slots).
 __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin,
                                                          uint32_t* __restrict__ kout,
@@ -124,9 +129,11 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
                                                          const uint32_t* __restrict__ offsets,
                                                          uint32_t tiles, const uint32_t* __restrict__ partials,
                                                          uint32_t chunks) {
-    __shared__ uint32_t s_run[kMaxRadix];            // next free global slot per digit
-    __shared__ uint32_t s_wave[kWaves][kMaxRadix];   // per-wavefront count, then base
-    __shared__ uint32_t s_pre[kFusedChunks];         // exclusive prefix of the chunk totals
+    __shared__ uint32_t s_run[kMaxRadix];                   // global slot of each digit's first item
+    __shared__ uint32_t s_start[kMaxRadix];                 // tile-local slot of each digit's first item
+    __shared__ uint32_t s_cnt[kItems * kWaves][kMaxRadix];  // digit counts per (round, wavefront), then bases
+    __shared__ uint32_t s_key[kTile], s_val[kTile];         // the tile in sorted order
+    __shared__ uint32_t s_pre[kFusedChunks];                // exclusive prefix of the chunk totals
     __shared__ uint32_t s_tmp[kWaves];
     const uint32_t radix = 1u << bits, mask = radix - 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -151,50 +158,69 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
     for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
         const uint32_t w = d * tiles + blockIdx.x;
         s_run[d] = offsets[w] + (partials ? s_pre[w / kScanChunk] : 0u);
-#pragma unroll
-        for (int k = 0; k < kWaves; ++k) s_wave[k][d] = 0;
+#pragma unroll 8
+        for (int k = 0; k < kItems * kWaves; ++k) s_cnt[k][d] = 0;
     }
     __syncthreads();
+    // 1. ranks within the wavefront, counts per (round, wavefront, digit)
     const unsigned long long lt = (1ull << lane) - 1ull;
     const uint32_t base = blockIdx.x * kTile;
+    uint32_t key[kItems], val[kItems], rank[kItems];
+#pragma unroll
     for (int r = 0; r < kItems; ++r) {
         const uint32_t e = base + r * kThreads + threadIdx.x;
         const bool valid = e < n;
-        const uint32_t key = valid ? kin[e] : 0u;
-        const uint32_t val = valid ? vin[e] : 0u;
-        const uint32_t digit = (key >> shift) & mask;
-        // lanes of this wavefront holding the same digit
+        key[r] = valid ? kin[e] : 0u;
+        val[r] = valid ? vin[e] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const bool valid = base + r * kThreads + threadIdx.x < n;
+        const uint32_t digit = (key[r] >> shift) & mask;
         unsigned long long same = __ballot(valid);
         for (uint32_t b = 0; b < bits; ++b) {
             const bool bit = (digit >> b) & 1u;
             const unsigned long long set = __ballot(bit);
             same &= bit ? set : ~set;
         }
-        const uint32_t rank = (uint32_t)__popcll(same & lt);
-        if (valid && rank == 0) s_wave[wave][digit] = (uint32_t)__popcll(same);
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
-            uint32_t run = s_run[d];
+        rank[r] = (uint32_t)__popcll(same & lt);
+        if (valid && rank[r] == 0) s_cnt[r * kWaves + wave][digit] = (uint32_t)__popcll(same);
+    }
+    __syncthreads();
+    // 2. per digit, the counts scanned in element order (round, then wavefront); then
+    // the digits' totals scanned into tile-local starts
+    uint32_t total = 0;
+    if (threadIdx.x < radix) {
+        const uint32_t d = threadIdx.x;
+#pragma unroll 4
+        for (int k = 0; k < kItems * kWaves; ++k) {
+            const uint32_t c = s_cnt[k][d];
+            s_cnt[k][d] = total;
+            total += c;
+        }
+    }
+    uint32_t all;
+    const uint32_t start = block_exclusive_scan(total, s_tmp, all);
+    if (threadIdx.x < radix) s_start[threadIdx.x] = start;
+    __syncthreads();
+    // 3. the tile in sorted order in LDS
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = s_wave[w][d];
-                s_wave[w][d] = run;
-                run += c;
-            }
-            s_run[d] = run;
+    for (int r = 0; r < kItems; ++r) {
+        if (base + r * kThreads + threadIdx.x < n) {
+            const uint32_t digit = (key[r] >> shift) & mask;
+            const uint32_t at = s_start[digit] + s_cnt[r * kWaves + wave][digit] + rank[r];
+            s_key[at] = key[r];
+            s_val[at] = val[r];
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t dst = s_wave[wave][digit] + rank;
-            kout[dst] = key;
-            vout[dst] = val;
-        }
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < radix; d += kThreads) {
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) s_wave[w][d] = 0;
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+    // 4. runs of one digit to consecutive global slots
+    const uint32_t count = n - base < (uint32_t)kTile ? n - base : (uint32_t)kTile;
+    for (uint32_t i = threadIdx.x; i < count; i += kThreads) {
+        const uint32_t k = s_key[i], digit = (k >> shift) & mask;
+        const uint32_t dst = s_run[digit] + (i - s_start[digit]);
+        kout[dst] = k;
+        vout[dst] = s_val[i];
     }
 }
 
