@@ -399,6 +399,10 @@ def main() -> int:
             results[form] = r
             print(f"bench: {form} {json.dumps({k: v for k, v in r.items() if k != 'stage_times'})}",
                   file=sys.stderr, flush=True)
+            for name, q in (("plain", plain), ("warm", r)):
+                if q.get("stage_times"):
+                    print(f"bench: {form} {name} stage_times {json.dumps(q['stage_times'])}", file=sys.stderr,
+                          flush=True)
             continue
         if form in ("batch_64", "batch_256", "batch_256_warm"):
             batch = scenes.batch_64() if form == "batch_64" else scenes.batch_256()

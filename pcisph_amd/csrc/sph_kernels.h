@@ -1566,6 +1566,73 @@ __global__ void k_warm_apply(Batch<R> B, const V4<R>* __restrict__ xr, const V4<
 // sharing the candidate loop.  Each scene continues with whichever start has the
 // smaller residual max |rho* - rho0| (k_select_start), so a prediction that makes
 // the residual worse is dropped and the loop is exactly the plain one.
+// The dual pass's results of fluid particle i from its two sums (sw from the
+// network's start, sz from the plain start): density variations, pressures and
+// p/rho^2 of both, and the maxima m (dv warm, expansion warm, dv plain, expansion plain).
+template <typename R, bool FAST>
+__device__ __forceinline__ void dual_fluid_outputs(const SphParams<R>& P, uint32_t i, const V4<R>& self, R sw, R sz,
+                                                   R* __restrict__ dv, R* __restrict__ p, V4<R>* __restrict__ xf,
+                                                   const DualArrays<R>& D, unsigned long long (&m)[4]) {
+    using O = Ops<R, FAST>;
+    sw *= P.density_scale;
+    sz *= P.density_scale;
+    const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
+    m[0] = Bits<R>::of(vw);
+    const bool bulk = self.w >= P.rho0;  // as in k_pressure: no free-surface particles
+    m[1] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sw)) : 0ull;
+    m[2] = Bits<R>::of(vz);
+    m[3] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sz)) : 0ull;
+    dv[i] = vw;
+    D.dv0[i] = vz;
+    R pi = p[i];
+    if (vw != R(0)) {
+        pi += vw * P.delta;
+        p[i] = pi;
+    }
+    const R pz0 = vz != R(0) ? R(0) + vz * P.delta : R(0);
+    D.p0[i] = pz0;
+    V4<R> out = self;
+    out.w = O::div(pi, self.w * self.w);
+    xf[i] = out;
+    out.w = O::div(pz0, self.w * self.w);
+    D.xf0[i] = out;
+}
+
+// The dual pass of boundary particle b (boundary mode, pcisph.cl:231-236, for both
+// starts).
+template <typename R, bool FAST>
+__device__ __forceinline__ void dual_boundary_particle(const SphParams<R>& P, uint32_t b, const CellTable& FT,
+                                                       const V4<R>* __restrict__ bpos, const R* __restrict__ binit,
+                                                       const V4<R>* __restrict__ xp, R* __restrict__ bp,
+                                                       V4<R>* __restrict__ bxf, const DualArrays<R>& D) {
+    using O = Ops<R, FAST>;
+    const V4<R> self = bpos[b];
+    const int3 c = cell_of4<R>(self, P.cell);
+    R sw = 0, sz = 0;
+    sw += binit[b];
+    sz += binit[b];
+    struct Pair {
+        V4<R> w, z;
+    };
+    walk_buckets<R>(FT, c, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
+                    [&](const Pair& o, uint32_t) {
+                        sw = poly6_add<R, FAST>(sw, self, o.w, P.h2);
+                        sz = poly6_add<R, FAST>(sz, self, o.z, P.h2);
+                    });
+    sw *= P.density_scale;
+    sz *= P.density_scale;
+    const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
+    const R pbw = vw != R(0) ? bp[b] + vw * P.delta : bp[b];
+    const R pbz = vz != R(0) ? R(0) + vz * P.delta : R(0);
+    bp[b] = pbw;
+    D.bp0[b] = pbz;
+    V4<R> out = self;
+    out.w = O::div(pbw, P.rho0_sq);
+    bxf[b] = out;
+    out.w = O::div(pbz, P.rho0_sq);
+    D.bxf0[b] = out;
+}
+
 template <typename R, bool FAST>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, const uint2* __restrict__ bcells, const V4<R>* __restrict__ bpos,
                                 const R* __restrict__ binit, const uint2* __restrict__ fcells,
@@ -1610,56 +1677,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual(Batch<R> B, 
                                 sw = poly6_add<R, FAST>(sw, pw, o.w, P.h2);
                                 sz = poly6_add<R, FAST>(sz, pz, o.z, P.h2);
                             });
-            sw *= P.density_scale;
-            sz *= P.density_scale;
-            const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
-            m[0] = Bits<R>::of(vw);
-            const bool bulk = self.w >= P.rho0;  // as in k_pressure: no free-surface particles
-            m[1] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sw)) : 0ull;
-            m[2] = Bits<R>::of(vz);
-            m[3] = bulk ? Bits<R>::of(fmax(R(0), P.rho0 - sz)) : 0ull;
-            dv[i] = vw;
-            D.dv0[i] = vz;
-            R pi = p[i];
-            if (vw != R(0)) {
-                pi += vw * P.delta;
-                p[i] = pi;
-            }
-            const R pz0 = vz != R(0) ? R(0) + vz * P.delta : R(0);
-            D.p0[i] = pz0;
-            V4<R> out = self;
-            out.w = O::div(pi, self.w * self.w);
-            xf[i] = out;
-            out.w = O::div(pz0, self.w * self.w);
-            D.xf0[i] = out;
+            dual_fluid_outputs<R, FAST>(P, i, self, sw, sz, dv, p, xf, D, m);
         }
     } else if (i < rg.end) {
-        const uint32_t b = i;
-        const V4<R> self = bpos[b];
-        const int3 c = cell_of4<R>(self, P.cell);
-        R sw = 0, sz = 0;
-        sw += binit[b];
-        sz += binit[b];
-        struct Pair {
-            V4<R> w, z;
-        };
-        walk_buckets<R>(FT, c, [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; },
-                        [&](const Pair& o, uint32_t) {
-                            sw = poly6_add<R, FAST>(sw, self, o.w, P.h2);
-                            sz = poly6_add<R, FAST>(sz, self, o.z, P.h2);
-                        });
-        sw *= P.density_scale;
-        sz *= P.density_scale;
-        const R vw = fmax(R(0), sw - P.rho0), vz = fmax(R(0), sz - P.rho0);
-        const R pbw = vw != R(0) ? bp[b] + vw * P.delta : bp[b];
-        const R pbz = vz != R(0) ? R(0) + vz * P.delta : R(0);
-        bp[b] = pbw;
-        D.bp0[b] = pbz;
-        V4<R> out = self;
-        out.w = O::div(pbw, P.rho0_sq);
-        bxf[b] = out;
-        out.w = O::div(pbz, P.rho0_sq);
-        D.bxf0[b] = out;
+        dual_boundary_particle<R, FAST>(P, i, FT, bpos, binit, xp, bp, bxf, D);
     }
     // 0: dv warm, 1: expansion warm, 2: dv plain, 3: expansion plain
     unsigned long long* const dst[4] = {&st->dual_dv[0], &st->dual_exp[0], &st->dual_dv[1], &st->dual_exp[1]};

@@ -116,8 +116,18 @@ void SphLaunch<R, FAST>::pressure_dual(const Launch& L, const DeviceArrays<R>& A
     const Batch<R>& B = A.batch;
     const uint32_t grid = B.fluid_blocks + B.boundary_blocks;
     if (!grid) return;
-    hipLaunchKernelGGL((k_pressure_dual<R, FAST>), dim3(grid), dim3(L.block), 0, L.stream, B, A.bcells, A.bpos,
-                       A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp, A.bxf, D);
+    bool done = false;
+    if constexpr (std::is_same<R, float>::value) {
+        if (A.staged_pressure) {
+            // fluid ranges from the step's LDS staging
+            hipLaunchKernelGGL((k_pressure_dual_staged<FAST>), dim3(grid), dim3(L.block), 0, L.stream, B, A.bcells,
+                               A.bpos, A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp, A.bxf, D, A.stage);
+            done = true;
+        }
+    }
+    if (!done)
+        hipLaunchKernelGGL((k_pressure_dual<R, FAST>), dim3(grid), dim3(L.block), 0, L.stream, B, A.bcells, A.bpos,
+                           A.binit, A.cells, A.xr, A.xp, A.dv, A.p, A.xf, A.bp, A.bxf, D);
     hipLaunchKernelGGL((k_select_start<R>), dim3(grid_for(B.scene_count, 256)), dim3(256), 0, L.stream, B.scenes,
                        B.states, B.scene_count);
     hipLaunchKernelGGL((k_adopt_plain_start<R>), dim3(grid), dim3(L.block), 0, L.stream, B, D, A.xp, A.xf, A.p,

@@ -501,6 +501,158 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_density_staged(
     }
 }
 
+// The poly6 terms of one staged row for both starts of the dual pass: candidates
+// [a, a + n) of the network start's staging (xyw, zw) against pw and of the plain
+// start's (xy0, z0) against p0, added to s.x and s.y.
+template <bool FAST>
+__device__ __forceinline__ float2 staged_row_dual(const float2* __restrict__ xyw, const float* __restrict__ zw,
+                                                  const float2* __restrict__ xy0, const float* __restrict__ z0,
+                                                  uint32_t a, uint32_t n, const float4& pw, const float4& p0,
+                                                  float h2, float2 s) {
+    const float2* const end4 = xyw + a + (n & ~3u);
+    const float2* const end = xyw + a + n;
+    const float2 *q = xyw + a, *r = xy0 + a;
+    const float *qz = zw + a, *rz = z0 + a;
+    for (; q != end4; q += 4, r += 4, qz += 4, rz += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s.x = poly6_add<float, FAST>(s.x, pw, make_float4(q[u].x, q[u].y, qz[u], 0.f), h2);
+            s.y = poly6_add<float, FAST>(s.y, p0, make_float4(r[u].x, r[u].y, rz[u], 0.f), h2);
+        }
+    }
+    for (; q != end; ++q, ++r, ++qz, ++rz) {
+        s.x = poly6_add<float, FAST>(s.x, pw, make_float4(q->x, q->y, *qz, 0.f), h2);
+        s.y = poly6_add<float, FAST>(s.y, p0, make_float4(r->x, r->y, *rz, 0.f), h2);
+    }
+    return s;
+}
+
+// The dual first iteration of a warm-started step (k_pressure_dual: both starts'
+// pressure passes in one) for the FP32 step, its fluid ranges from the step's LDS
+// staging: both predictions of every staged fluid candidate in LDS (48 KB at the
+// staging cap), the boundary candidates after them, each lane's nine rows walked once
+// for both sums in the reference's order -- the global walk's sums bit for bit.
+// Partly staged and unstaged workgroups walk global memory as k_pressure_dual does;
+// boundary ranges as k_pressure_dual.
+template <bool FAST>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pressure_dual_staged(
+    Batch<float> B, const uint2* __restrict__ bcells, const float4* __restrict__ bpos, const float* __restrict__ binit,
+    const uint2* __restrict__ fcells, const float4* __restrict__ xr, const float4* __restrict__ xp,
+    float* __restrict__ dv, float* __restrict__ p, float4* __restrict__ xf, float* __restrict__ bp,
+    float4* __restrict__ bxf, DualArrays<float> D, StageMaps S) {
+    __shared__ float2 s_xyw[kStageCap];
+    __shared__ float s_zw[kStageCap];
+    __shared__ float2 s_xy0[kStageCap];
+    __shared__ float s_z0[kStageCap];
+    __shared__ unsigned long long lds[4][8];
+    if (D.disp_next) clear_table(D.disp_next, D.buckets);  // for iteration 1's prediction
+    const BlockRange rg = B.blocks[blockIdx.x];
+    const SphParams<float> P = B.scenes[rg.scene];
+    if (!P.warm) return;  // scenes left out of the warm start: the plain pass runs iteration 0
+    LoopState* st = B.states + rg.scene;
+    const CellTable FT = cell_table(P, fcells, B.fgrid);
+    unsigned long long m[4] = {0, 0, 0, 0};  // dv warm, exp warm, dv plain, exp plain
+    struct Pair {
+        float4 w, z;
+    };
+    if (rg.flags & kRangeBoundary) {
+        const uint32_t t = rg.begin + threadIdx.x;
+        if (t < rg.end) dual_boundary_particle<float, FAST>(P, t, FT, bpos, binit, xp, bp, bxf, D);
+    } else {
+        const uint32_t tile = rg.flags >> kRangeTileShift;
+        const uint32_t total = S.info[tile];
+        const bool staged = total != kStageFallback;
+        const uint32_t btotal = staged ? S.binfo[tile] : kStageFallback;
+        const bool bstaged = btotal != kStageFallback;
+        const uint32_t fitted = staged ? min(total, S.cap) : 0u;
+        if (staged) {
+            const uint32_t all = fitted + (bstaged ? btotal : 0u);
+            const uint32_t* src = S.src + (size_t)tile * kStageCap;
+            for (uint32_t q0 = threadIdx.x; q0 < all; q0 += kCopyBatch * blockDim.x) {
+                uint32_t j[kCopyBatch];
+                float4 v[kCopyBatch], v0[kCopyBatch];
+#pragma unroll
+                for (int u = 0; u < kCopyBatch; ++u) {
+                    const uint32_t q = q0 + u * blockDim.x;
+                    j[u] = src[q < all ? q : q0];
+                }
+#pragma unroll
+                for (int u = 0; u < kCopyBatch; ++u) {
+                    const uint32_t q = q0 + u * blockDim.x, qq = q < all ? q : q0;
+                    if (qq < fitted) {
+                        v[u] = xp[j[u]];
+                        v0[u] = D.xp0[j[u]];
+                    } else {
+                        v[u] = v0[u] = bpos[j[u]];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kCopyBatch; ++u) {
+                    const uint32_t q = q0 + u * blockDim.x;
+                    if (q < all) {
+                        s_xyw[q] = make_float2(v[u].x, v[u].y);
+                        s_zw[q] = v[u].z;
+                        s_xy0[q] = make_float2(v0[u].x, v0[u].y);
+                        s_z0[q] = v0[u].z;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t k = rg.begin + threadIdx.x;
+        if (k < rg.end) {
+            const uint32_t i = S.work[k];
+            const float4 self = xr[i];
+            const float4 pw = xp[i], pz = D.xp0[i];
+            const int3 c = cell_of4<float>(self, P.cell);
+            float2 sum = make_float2(0.f, 0.f);
+            auto both = [&](const float4& o, uint32_t) {
+                sum.x = poly6_add<float, FAST>(sum.x, pw, o, P.h2);
+                sum.y = poly6_add<float, FAST>(sum.y, pz, o, P.h2);
+            };
+            auto pair = [&](const Pair& o, uint32_t) {
+                sum.x = poly6_add<float, FAST>(sum.x, pw, o.w, P.h2);
+                sum.y = poly6_add<float, FAST>(sum.y, pz, o.z, P.h2);
+            };
+            auto pload = [&](uint32_t j) { return Pair{xp[j], D.xp0[j]}; };
+            if (P.nb) {
+                CellTable BT = cell_table(P, bcells, B.bgrid);
+                BT.prune = far_prunable<float>(displacement<float>(pw, self), (float)P.h, P.cell) &&
+                           far_prunable<float>(displacement<float>(pz, self), (float)P.h, P.cell);
+                if (BT.prune && quiet_cell(BT, c)) {
+                    // nothing within reach
+                } else if (BT.prune && bstaged) {
+                    if (btotal) {
+#pragma unroll 1
+                        for (int row = 0; row < 9; ++row) {
+                            const uint32_t d = S.brows[(size_t)row * B.n_total + k];
+                            sum = staged_row_dual<FAST>(s_xyw, s_zw, s_xyw, s_zw, d & 0xFFFFu, d >> 16, pw, pz,
+                                                        P.h2, sum);
+                        }
+                    }
+                } else {
+                    walk_buckets<float>(BT, c, [&](uint32_t j) { return bpos[j]; }, both);
+                }
+            }
+            if (staged) {
+#pragma unroll 1
+                for (int row = 0; row < 9; ++row) {
+                    const uint32_t d = S.rows[(size_t)row * B.n_total + k];
+                    const uint32_t a = d & 0xFFFFu, n = d >> 16;
+                    if (a + n <= fitted) sum = staged_row_dual<FAST>(s_xyw, s_zw, s_xy0, s_z0, a, n, pw, pz, P.h2, sum);
+                    else global_row_each(FT, c, row, pload, pair);
+                }
+            } else {
+                walk_buckets<float>(FT, c, pload, pair);
+            }
+            dual_fluid_outputs<float, FAST>(P, i, self, sum.x, sum.y, dv, p, xf, D, m);
+        }
+    }
+    // 0: dv warm, 1: expansion warm, 2: dv plain, 3: expansion plain
+    unsigned long long* const dst[4] = {&st->dual_dv[0], &st->dual_exp[0], &st->dual_dv[1], &st->dual_exp[1]};
+    block_max_to<4>(m, dst, lds);
+}
+
 // update_pressure (pcisph.cl:212-263), the fluid ranges with their candidates from the
 // workgroup's LDS staging (see above) -- the same results as k_pressure's bucket walk --
 // and the boundary ranges (boundary mode) walking their fluid buckets in global

@@ -388,6 +388,7 @@ def test_autotune_restores_warm_start_eligibility(gpu):
 
 # ---------------------------------------------------------------- staged pressure pass
 STAGED = 3 | 16  # default work order plus the LDS-staged FP32 pressure pass in every step
+STAGED_OFF = 3 | 32  # default work order, the LDS staging off
 
 
 @pytest.mark.parametrize("scene_idx", [0, 1])
@@ -491,3 +492,29 @@ def test_staged_batch_matches_oracle(gpu, oracle_lib):
         g.select_scene(i)
         assert_same_state(g, o, tag=sc.name)
     g.close()
+
+
+@pytest.mark.parametrize("precision", ["ieee", "fast"])
+@pytest.mark.parametrize("cap", [0, 256])
+def test_staged_dual_pass_equals_global(gpu, precision, cap):
+    """The dual first iteration of a warm-started step (both starts' pressure passes
+    in one) from the LDS staging, fully and partly staged, leaves the bits of the
+    global-memory dual pass: the committed trained network, whose start the scene keeps
+    (so every step runs the dual pass and continues from the network's start), three
+    steps with the staging forced on and off."""
+    params = np.load(os.path.join(os.path.dirname(os.path.dirname(__file__)), "pcisph_amd", "data",
+                                  "warm_start.npy"))
+    scene = small_scenes()[1]
+    prec = gpu.PRECISION_IEEE if precision == "ieee" else gpu.PRECISION_FAST
+    print(f"case staged dual pass {precision} cap {cap}", flush=True)
+    runs = []
+    for order in (STAGED, STAGED_OFF):
+        g = gpu.Fluid(precision=prec)
+        g.mlp_set_params(params)
+        g.set_warm_start(True)
+        runs.append((g, run(g, scene, 3, max_iterations=20, work_order=order, stage_cap=cap)))
+    (gs, ss), (gg, sg) = runs
+    assert any(s["warm_start"] & 3 == 1 for s in ss), ss  # the network's start ran and was kept
+    assert [s["warm_start"] for s in ss] == [s["warm_start"] for s in sg]
+    assert_same_stats(ss, sg, f"dual {precision} cap {cap}")
+    assert_same_state(gs, gg, tag=f"dual {precision} cap {cap}")
