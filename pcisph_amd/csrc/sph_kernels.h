@@ -470,23 +470,35 @@ __device__ __forceinline__ void unpack_chunk(const uint3 w, uint32_t (&e)[4]) {
     e[3] = w.z >> 8;
 }
 
+// The chunk being filled is kept as one 96-bit word w (w[0] lowest) that each push
+// shifts right by 24 bits, entering the new entry at the top: after four pushes it
+// holds pack_chunk of them, oldest lowest (three funnel shifts a push, where placing an
+// entry by its position in the chunk costs a compare and a select per entry).
 struct ListWriter {
     uint32_t* base;
     uint32_t stride, i, cap, len;
-    uint32_t e[4];
+    uint32_t w[3];
+    __device__ __forceinline__ void shift_in(uint32_t j) {
+        w[0] = __builtin_amdgcn_alignbit(w[1], w[0], 24u);
+        w[1] = __builtin_amdgcn_alignbit(w[2], w[1], 24u);
+        w[2] = __builtin_amdgcn_alignbit(j, w[2], 24u);
+    }
     __device__ __forceinline__ void store_chunk() {
-        reinterpret_cast<uint3*>(base)[(size_t)(len >> 2) * stride + i] = pack_chunk(e[0], e[1], e[2], e[3]);
+        reinterpret_cast<uint3*>(base)[(size_t)(len >> 2) * stride + i] = make_uint3(w[0], w[1], w[2]);
     }
     __device__ __forceinline__ void push(uint32_t j) {
-        const uint32_t q = len & 3u;
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) e[u] = q == u ? j : e[u];
-        if (q == 3u && len < cap) store_chunk();
+        shift_in(j);
+        if ((len & 3u) == 3u && len < cap) store_chunk();
         ++len;
     }
-    // the last, partial chunk; then the length (kListOverflow past the capacity)
+    // the last, partial chunk (shifted down to the bottom); then the length
+    // (kListOverflow past the capacity)
     __device__ __forceinline__ uint32_t finish() {
-        if ((len & 3u) && len < cap) store_chunk();
+        const uint32_t r = len & 3u;
+        if (r && len < cap) {
+            for (uint32_t u = r; u < 4u; ++u) shift_in(0u);
+            store_chunk();
+        }
         return len <= cap ? len : kListOverflow;
     }
 };
