@@ -123,11 +123,14 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
                                                                   const uint2* __restrict__ bcells,
                                                                   const V4<R>* __restrict__ pos, StageMaps S) {
     __shared__ int s_lx[kStageLines], s_hx[kStageLines];  // x extent of each line
-    __shared__ uint32_t s_cell0[kStageLines + 1];         // first staged cell of each line
+    // 16-bit where the values that are read stay below 2^16 (cells <= kStageMaxCells,
+    // staged offsets <= 2 cap); counts are clamped to 0xFFFF, which still sends the
+    // workgroup to the fallback.  34 KB of LDS in all: four workgroups per CU.
+    __shared__ uint16_t s_cell0[kStageLines + 1];         // first staged cell of each line
     __shared__ uint32_t s_start[kStageMaxCells];          // bucket start of each staged cell
-    __shared__ uint32_t s_off[kStageMaxCells + 1];        // staged offset of each cell (count before the scan)
+    __shared__ uint16_t s_off[kStageMaxCells + 1];        // staged offset of each cell (count before the scan)
     __shared__ uint32_t s_bstart[kStageMaxCells];         // the same for the boundary buckets
-    __shared__ uint32_t s_boff[kStageMaxCells + 1];
+    __shared__ uint16_t s_boff[kStageMaxCells + 1];
     __shared__ uint32_t s_part[2][kMaxBlockThreads / 64];
     __shared__ int s_box[6][kMaxBlockThreads / 64];
     const BlockRange rg = B.blocks[blockIdx.x];
@@ -194,15 +197,20 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
         const uint32_t chunk = (nl + blockDim.x - 1) / blockDim.x;
         const uint32_t l0 = min(threadIdx.x * chunk, nl), l1 = min(l0 + chunk, nl);
         uint32_t sum = 0;
-        for (uint32_t l = l0; l < l1; ++l) sum += s_hx[l] >= s_lx[l] ? (uint32_t)(s_hx[l] - s_lx[l] + 1) : 0u;
+        // a line of more than kStageMaxCells cells counts as kStageMaxCells + 1 (it sends
+        // the workgroup to the fallback either way), so the sums cannot wrap
+        auto cells_of = [&](uint32_t l) {
+            return s_hx[l] >= s_lx[l] ? min((uint32_t)(s_hx[l] - s_lx[l]) + 1u, kStageMaxCells + 1u) : 0u;
+        };
+        for (uint32_t l = l0; l < l1; ++l) sum += cells_of(l);
         uint2 tot;
         const uint2 base = block_scan2(sum, 0u, tot, s_part);
         uint32_t run = base.x;
         for (uint32_t l = l0; l < l1; ++l) {
-            s_cell0[l] = run;
-            run += s_hx[l] >= s_lx[l] ? (uint32_t)(s_hx[l] - s_lx[l] + 1) : 0u;
+            s_cell0[l] = (uint16_t)run;
+            run += cells_of(l);
         }
-        if (threadIdx.x == 0) s_cell0[nl] = tot.x;
+        if (threadIdx.x == 0) s_cell0[nl] = (uint16_t)min(tot.x, 0xFFFFu);
         if (tot.x > kStageMaxCells) {
             if (threadIdx.x == 0) S.info[tile] = S.binfo[tile] = kStageFallback;
             return;
@@ -236,9 +244,9 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
             const uint32_t q = q0 + u * blockDim.x;
             if (q >= ncells) continue;
             s_start[q] = r[u].x;
-            s_off[q] = r[u].y - r[u].x;
+            s_off[q] = (uint16_t)min(r[u].y - r[u].x, 0xFFFFu);
             s_bstart[q] = rb[u].x;
-            s_boff[q] = rb[u].y - rb[u].x;
+            s_boff[q] = (uint16_t)min(rb[u].y - rb[u].x, 0xFFFFu);
         }
     }
     __syncthreads();
@@ -253,8 +261,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
         uint32_t run = base.x, brun = base.y;
         for (uint32_t q = q0; q < q1; ++q) {
             const uint32_t cnt = s_off[q], bcnt = s_boff[q];
-            s_off[q] = run;
-            s_boff[q] = brun;
+            s_off[q] = (uint16_t)run;
+            s_boff[q] = (uint16_t)brun;
             run += cnt;
             brun += bcnt;
         }
@@ -263,8 +271,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage_setup(Batch<R> B, co
     // the boundary candidates follow the fluid ones in the staging, if both fit
     const uint32_t btotal = total + tot.y <= S.cap ? tot.y : kStageFallback;
     if (threadIdx.x == 0) {
-        s_off[ncells] = total;
-        s_boff[ncells] = tot.y;
+        s_off[ncells] = (uint16_t)total;
+        s_boff[ncells] = (uint16_t)tot.y;
     }
     if (total > 2u * S.cap) {
         if (threadIdx.x == 0) S.info[tile] = S.binfo[tile] = kStageFallback;
